@@ -1,0 +1,206 @@
+"""Benchmark: closest-hit ray casting on one MI355X.
+
+Workload (one "step" = one closest-hit pass over one batch resident in HBM):
+2,097,152-triangle fBm heightfield (1024 x 1024 cells), octree split 16 /
+depth 12, batch B = 4,194,304 incoherent rays (origins uniform in the scene
+AABB, directions uniform on S^2), clipBackfaces = false.  The coherent 1080p
+primary batch (A) is timed beside it.
+
+Prints ONE JSON line (rank 0) with the metric, a roofline object for the
+traversal kernel (HIP events on the launch stream) and the oracle CPU path
+timed on this host as cpu_baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0                      # MI355X HBM3E
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # lane-ops/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
+# algorithmic work per ray (reference algorithm, counted by the oracle on the same rays):
+# bytes = 24 ray in + 8 (id, t) out + 28 per box (6 fp32 bounds + link) + 4 per leaf-list entry
+#         + 36 per triangle test (9 fp32 vertices); ops = 6 + 24 per box + 155 per triangle test
+B_RAY, B_OUT, B_BOX, B_REF, B_TRI = 24, 8, 28, 4, 36
+OPS_RAY, OPS_BOX, OPS_TRI = 6, 24, 155
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(path):
+    """Measured HBM bytes per launch from a committed rocprofv3 --pmc summary (or None)."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--rays", type=int, default=1 << 22)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (pmc calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (each GPU box has one MI355X); "
+              "multi-GPU sharding is described in DESIGN.md but not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dumbray_amd as D
+    from dumbray_amd import scenes as S
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("generating workload")
+    tris, coherent, incoherent, cam = S.raycast_workload(cells=args.cells, n_incoherent=args.rays)
+    t0 = time.perf_counter()
+    tree = D.Octree(16, 12)
+    tree.push(tris)
+    tree.build()
+    build_s = time.perf_counter() - t0
+    sizes = tree.device_layout_sizes()
+    scene = tree.upload()
+    log(f"tris={len(tris)} nodes={sizes['nodes']} groups={sizes['groups']} records={sizes['records']} "
+        f"build={build_s:.2f}s device_bytes={scene.device_bytes()}")
+    rB = torch.from_numpy(incoherent).to(dev)
+    rA = torch.from_numpy(coherent).to(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step(r):
+        return scene.cast(r, clip_backfaces=False, stream=stream)
+
+    for _ in range(args.warmup):
+        step(rB)
+        step(rA)
+    torch.cuda.synchronize()
+
+    def timed(r, k):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(k):
+            out = step(r)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        return e0.elapsed_time(e1) / k, wall, out
+
+    kB_ms, wallB, outB = timed(rB, args.steps)
+    log(f"batch B: {kB_ms:.3f} ms/launch ({len(incoherent) / kB_ms / 1e3:.1f} Mrays/s)")
+    kA_ms, wallA, outA = timed(rA, args.steps)
+    log(f"batch A: {kA_ms:.3f} ms/launch ({len(coherent) / kA_ms / 1e3:.1f} Mrays/s)")
+    ms_per_step = wallB * 1e3 / args.steps
+    value = len(incoherent) / (ms_per_step / 1e3) / 1e6
+
+    if args.calib_copy:
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 bytes read + written")
+
+    # ---------------------------------------------------------------- oracle leg
+    per_ray = None
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle
+        oracle.build()
+        log("oracle: building the reference octree on the host")
+        ot = oracle.Octree(tris, 16, 12)
+        threads = min(os.cpu_count() or 1, 16)
+        rng = np.random.default_rng(11)
+        idx = np.sort(rng.choice(len(incoherent), 65536, replace=False))
+        ores = ot.cast(incoherent[idx], counters=True, nthreads=threads)
+        gid = outB["id"].cpu().numpy()[idx]
+        parity = int(np.sum(gid != ores["id"]))
+        nb, nr, nt = (float(ores[k].mean()) for k in ("n_box", "n_leafref", "n_tri"))
+        per_ray = {"n_box": nb, "n_leafref": nr, "n_tri": nt,
+                   "bytes": B_RAY + B_OUT + B_BOX * nb + B_REF * nr + B_TRI * nt,
+                   "ops": OPS_RAY + OPS_BOX * nb + OPS_TRI * nt}
+        log(f"oracle sample parity mismatches={parity}; per ray boxes={nb:.1f} refs={nr:.1f} tris={nt:.1f}")
+        # CPU baseline: oracle cast on a bounded sample sized to ~cpu_seconds
+        probe = incoherent[:20000]
+        t0 = time.perf_counter()
+        ot.cast(probe, nthreads=threads)
+        rate = len(probe) / (time.perf_counter() - t0)
+        ns = int(min(len(incoherent), max(20000, rate * args.cpu_seconds)))
+        sample = incoherent[:ns]
+        t0 = time.perf_counter()
+        ot.cast(sample, nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(ns / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"first {ns} of the 4,194,304 incoherent rays, oracle Octree::cast, {threads} threads, "
+                         f"{cpu_s:.1f} s"}
+        log(f"cpu baseline: {cpu['value']} Mrays/s on {threads} threads")
+
+    roofline = None
+    valu = None
+    if per_ray is not None:
+        algo_bytes = per_ray["bytes"] * len(incoherent)
+        achieved = algo_bytes / (kB_ms / 1e3) / 1e9
+        traffic = read_traffic(os.path.join(ROOT, "profiles", "cast_incoherent_hf2M_traffic.txt"))
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": "dr::k_cast", "kernel_ms": round(kB_ms, 4),
+                    "algo_bytes_per_launch": int(algo_bytes)}
+        ops = per_ray["ops"] * len(incoherent)
+        valu = {"achieved": round(ops / (kB_ms / 1e3) / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "T lane-ops/s", "frac": round(ops / (kB_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)}
+
+    line = {
+        "metric": "closest-hit Mrays/s & Mpix·spp/s at 1080p; adjoint/forward time ratio",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"closest-hit cast, {len(incoherent)} incoherent rays (batch B) on a "
+                               f"{args.cells}x{args.cells} fBm heightfield ({len(tris)} tris)",
+                   "octree": "split 16 / depth 12", "clip_backfaces": False, "parallelism": "single GPU"},
+        "coherent_1080p_mrays_s": round(len(coherent) / kA_ms / 1e3, 2),
+        "incoherent_kernel_mrays_s": round(len(incoherent) / kB_ms / 1e3, 2),
+        "octree_build_s": round(build_s, 3),
+        "roofline": roofline,
+        "valu": valu,
+        "per_ray_work": per_ray,
+        "oracle_sample_mismatches": parity,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,39 @@
+// Internal helpers shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include "../../include/dumbray.h"
+#include "dr_octree_build.h"
+#include "dr_traverse.h"
+
+namespace dr {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+#define DR_HIP_CHECK(expr)                                                                            \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) return ::dr::fail(DR_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device buffer owned by a scene.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+int dev_upload(DevBuf &b, const void *host, size_t bytes);
+void dev_free(DevBuf &b);
+
+}  // namespace dr
+
+struct dr_octree_s {
+  dr::OctreeHost host;
+};
+
+struct dr_scene_s {
+  int device = 0;
+  dr::DevBuf groups, recs;
+  dr::DevTree tree;
+  int max_depth = 0;
+  int64_t bytes() const { return (int64_t)(groups.bytes + recs.bytes); }
+};

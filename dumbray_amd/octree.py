@@ -1,0 +1,155 @@
+"""Host mirror of the reference's geometry container, over the C ABI.
+
+``Octree`` keeps the shape of ``Octree<Renderable<BakedTriFace>>``
+(Octree.cuh:50-228): ``reset``/``push``/``build``/``getNodeCount``/``getData``
+and ``cast``.  ``push`` collects triangles in order (their index is the hit id,
+Octree.impl.cuh:97-111); ``build`` runs the native parallel builder
+(dr_octree_build); ``upload`` makes the device-resident scene (replacing the
+deep copy of TypeTools<Octree>::prepareForCpyLoad, Octree.impl.cuh:763-796);
+``cast`` is batch ``Octree::cast`` (Octree.impl.cuh:174-212) on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import DumbRayError, check, lib
+
+OCTREE_POLYCOUNT_TO_SPLIT_NODE = 8   # Octree.cuh:11
+OCTREE_MAX_DEPTH = 24                # Octree.cuh:13
+DR_CAST_CLIP_BACKFACES = 1
+
+
+def _vp(x):
+    return ctypes.c_void_p(x)
+
+
+class Octree:
+    def __init__(self, split_threshold: int = OCTREE_POLYCOUNT_TO_SPLIT_NODE, max_depth: int = OCTREE_MAX_DEPTH):
+        self.split_threshold = int(split_threshold)
+        self.max_depth = int(max_depth)
+        self._parts: list[np.ndarray] = []
+        self._data = np.zeros((0, 9), np.float32)
+        self._h = None
+
+    # ------------------------------------------------------------ push & build
+    def reset(self):
+        self._free()
+        self._parts = []
+        self._data = np.zeros((0, 9), np.float32)
+
+    def push(self, tris):
+        t = np.ascontiguousarray(np.asarray(tris, dtype=np.float32)).reshape(-1, 9)
+        self._parts.append(t)
+        self._free()
+
+    def build(self, nthreads: int = 0):
+        if self._parts:
+            self._data = np.ascontiguousarray(np.concatenate([self._data] + self._parts, axis=0))
+            self._parts = []
+        self._free()
+        h = ctypes.c_void_p()
+        rc = lib().dr_octree_build(self._data.ctypes.data_as(ctypes.c_void_p), int(self._data.shape[0]),
+                                   self.split_threshold, self.max_depth, int(nthreads), ctypes.byref(h))
+        check(rc, "dr_octree_build")
+        self._h = h
+        return self
+
+    def getData(self) -> np.ndarray:
+        return self._data
+
+    def _info(self):
+        self._require()
+        vals = [ctypes.c_int() for _ in range(4)]
+        check(lib().dr_octree_info(self._h, *[ctypes.byref(v) for v in vals]), "dr_octree_info")
+        return [v.value for v in vals]
+
+    def getNodeCount(self) -> int:
+        return self._info()[0]
+
+    def device_layout_sizes(self):
+        n, tot, groups, recs = self._info()
+        return {"nodes": n, "list_total": tot, "groups": groups, "records": recs}
+
+    def export(self):
+        """(bounds[n,6], children[n], list_off[n+1], list[total]) in breadth-first node order."""
+        n, tot, _, _ = self._info()
+        bounds = np.zeros((n, 6), np.float32)
+        children = np.zeros(n, np.int32)
+        off = np.zeros(n + 1, np.int32)
+        lst = np.zeros(max(tot, 1), np.int32)
+        check(lib().dr_octree_export(self._h, bounds.ctypes.data_as(ctypes.c_void_p),
+                                     children.ctypes.data_as(ctypes.c_void_p), off.ctypes.data_as(ctypes.c_void_p),
+                                     lst.ctypes.data_as(ctypes.c_void_p)), "dr_octree_export")
+        return bounds, children, off, lst[:tot]
+
+    def upload(self) -> "DeviceScene":
+        self._require()
+        return DeviceScene(self)
+
+    # ------------------------------------------------------------------ misc
+    def _require(self):
+        if self._h is None:
+            raise DumbRayError("Octree.build() has not been called")
+
+    def _free(self):
+        if self._h is not None:
+            lib().dr_octree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+
+class DeviceScene:
+    """Device-resident octree (+ shading tables once attached)."""
+
+    def __init__(self, octree: Octree):
+        h = ctypes.c_void_p()
+        check(lib().dr_scene_create(octree._h, ctypes.byref(h)), "dr_scene_create")
+        self._h = h
+        self.ntris = int(octree.getData().shape[0])
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        return int(lib().dr_scene_device_bytes(self._h))
+
+    def cast(self, rays, clip_backfaces: bool = True, exclude=None, points: bool = False, stream=None):
+        """Batch closest hit.  rays: torch float32 (n, 6) on the GPU.  Returns dict of
+        torch tensors id (int32, -1 = miss), t (float32, FLT_MAX on miss) [, point]."""
+        import torch
+
+        if not (isinstance(rays, torch.Tensor) and rays.is_cuda and rays.dtype == torch.float32):
+            raise DumbRayError("cast: rays must be a float32 GPU tensor of shape (n, 6)")
+        rays = rays.contiguous().view(-1, 6)
+        n = rays.shape[0]
+        ids = torch.empty(n, dtype=torch.int32, device=rays.device)
+        t = torch.empty(n, dtype=torch.float32, device=rays.device)
+        pts = torch.empty((n, 3), dtype=torch.float32, device=rays.device) if points else None
+        ex = None
+        if exclude is not None:
+            ex = exclude.contiguous().to(device=rays.device, dtype=torch.int32)
+        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        rc = lib().dr_cast(self._h, _vp(rays.data_ptr()), int(n), DR_CAST_CLIP_BACKFACES if clip_backfaces else 0,
+                           _vp(ex.data_ptr()) if ex is not None else None, _vp(ids.data_ptr()), _vp(t.data_ptr()),
+                           _vp(pts.data_ptr()) if pts is not None else None, _vp(s.cuda_stream))
+        check(rc, "dr_cast")
+        out = {"id": ids, "t": t}
+        if points:
+            out["point"] = pts
+        return out
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                lib().dr_scene_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

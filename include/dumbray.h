@@ -1,0 +1,82 @@
+/*
+ * dumbray_amd C ABI: the drop-in boundary of the MI355X path-tracing hot path.
+ *
+ * Plain C types only (pointers and sizes, no torch or HIP types in signatures).
+ * Every entry point returns an int status (DR_OK = 0) unless stated otherwise;
+ * dr_last_error() returns a thread-local message for the last failure.  No C++
+ * exception crosses this boundary.  The library owns device memory behind opaque
+ * handles; callers own host buffers and no host pointer is retained after a call
+ * returns.  Pointers documented as "device" are HBM addresses on the current
+ * device; `stream` is a hipStream_t passed as void* (NULL = null stream).
+ * Handles are immutable after creation and safe to use from several host
+ * threads on distinct streams.
+ */
+#ifndef DUMBRAY_H
+#define DUMBRAY_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DR_OK = 0,
+  DR_ERR_ARG = 1,      /* invalid argument / shape */
+  DR_ERR_HIP = 2,      /* HIP runtime error */
+  DR_ERR_ALLOC = 3,    /* out of memory */
+  DR_ERR_BUILD = 4     /* octree build failed */
+};
+
+/* Flags for dr_cast. */
+#define DR_CAST_CLIP_BACKFACES 1u   /* Octree::cast's clipBackfaces argument (Octree.cuh:130) */
+
+typedef struct dr_octree_s *dr_octree;   /* host-built octree (Octree<Renderable<BakedTriFace>>) */
+typedef struct dr_scene_s *dr_scene;     /* device-resident scene (geometry + shading tables) */
+
+/* Thread-local text of the last error (never NULL). */
+const char *dr_last_error(void);
+/* Library version string (never NULL). */
+const char *dr_version(void);
+
+/* Make `device` current for the calling thread.
+ * Replaces the per-GPU setup of BlockRenderer (BlockRenderer.cu:42-55). */
+int dr_device_open(int device);
+
+/* Octree<Renderable<BakedTriFace>>::push(all) + build()
+ * (Octree.impl.cuh:97-129 push/build, :339-533 split/reduce).
+ * tris: host, ntris x 9 floats (a, b, c); triangle i gets hit id i (push order,
+ * = index into Octree::getData()).  split_threshold = OCTREE_POLYCOUNT_TO_SPLIT_NODE,
+ * max_depth = OCTREE_MAX_DEPTH (Octree.cuh:11-13), both runtime here (max_depth <= 24).
+ * nthreads <= 0 uses every host core. */
+int dr_octree_build(const float *tris, int ntris, int split_threshold, int max_depth, int nthreads,
+                    dr_octree *out);
+/* Sizes: node count (Octree::getNodeCount, Octree.impl.cuh:219-221), total leaf-list
+ * entries, device groups and triangle records. Any pointer may be NULL. */
+int dr_octree_info(dr_octree t, int *node_count, int *list_total, int *ngroups, int *nrec);
+/* Nodes in breadth-first order (children of a node contiguous, reference child
+ * numbering): bounds[n*6] (start.xyz, end.xyz), children[n] (first child or -1),
+ * list_off[n+1], list[list_total] (triangle ids in stored order). Host buffers. */
+int dr_octree_export(dr_octree t, float *bounds, int32_t *children, int32_t *list_off, int32_t *list);
+void dr_octree_destroy(dr_octree t);
+
+/* Upload an octree to the current device (replaces TypeTools<Octree>::prepareForCpyLoad
+ * deep copy + pointer relocation, Octree.impl.cuh:763-796).  The scene keeps its own
+ * copy; the octree may be destroyed afterwards. */
+int dr_scene_create(dr_octree t, dr_scene *out);
+void dr_scene_destroy(dr_scene s);
+/* Device bytes held by the scene. */
+int64_t dr_scene_device_bytes(dr_scene s);
+
+/* Batch Octree::cast (Octree.impl.cuh:174-212) + validateNotSameAsObject (:163).
+ * rays: device, n x 6 floats (origin, direction; direction need not be unit).
+ * exclude: device int32[n] triangle id the ray must not hit, or NULL.
+ * out_id: device int32[n] hit id or -1; out_t: device float[n] hit distance
+ * (FLT_MAX on miss, Octree.impl.cuh:170); out_point: device float[n*3] or NULL.
+ * Asynchronous on `stream`. */
+int dr_cast(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude, int32_t *out_id,
+            float *out_t, float *out_point, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,193 @@
+"""ctypes binding of the CPU oracle (liboracle.so built from dr_oracle*.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker.  The dumbray_amd package
+never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE], check=True)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        L.ora_tri_cast.restype = ctypes.c_int
+        L.ora_tri_cast.argtypes = [F32P, F32P, F32P, ctypes.c_int, ctypes.POINTER(ctypes.c_float), F32P]
+        L.ora_tri_contains.restype = ctypes.c_int
+        L.ora_tri_contains.argtypes = [F32P, F32P]
+        L.ora_aabb_cast_preinv.restype = ctypes.c_float
+        L.ora_aabb_cast_preinv.argtypes = [F32P, F32P, F32P]
+        L.ora_aabb_intersects_tri.restype = ctypes.c_int
+        L.ora_aabb_intersects_tri.argtypes = [F32P, F32P]
+        L.ora_intersection_bounds.argtypes = [F32P, F32P, F32P]
+        L.ora_intersection_center.argtypes = [F32P, F32P, F32P]
+        L.ora_clip_tri.restype = ctypes.c_int
+        L.ora_clip_tri.argtypes = [F32P, F32P, F32P]
+        L.ora_tri_get_masses.argtypes = [F32P, F32P, F32P]
+        L.ora_tri_mass_center_w.argtypes = [F32P, F32P, F32P]
+        L.ora_octree_build.restype = vp
+        L.ora_octree_build.argtypes = [F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ora_octree_free.argtypes = [vp]
+        L.ora_octree_node_count.restype = ctypes.c_int
+        L.ora_octree_node_count.argtypes = [vp]
+        L.ora_octree_list_total.restype = ctypes.c_int
+        L.ora_octree_list_total.argtypes = [vp]
+        L.ora_octree_export.argtypes = [vp, F32P, I32P, I32P, I32P]
+        L.ora_octree_cast.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, vp, I32P, F32P, vp, vp, vp, vp,
+                                      ctypes.c_int]
+        L.ora_brute_cast.argtypes = [F32P, ctypes.c_int, F32P, ctypes.c_int, ctypes.c_int, vp, I32P, F32P, vp,
+                                     ctypes.c_int]
+        if hasattr(L, "ora_render"):
+            _bind_render(L)
+        _LIB = L
+    return _LIB
+
+
+def _bind_render(L):
+    vp = ctypes.c_void_p
+    L.ora_render.restype = ctypes.c_int
+    L.ora_render.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+    L.ora_backward.restype = ctypes.c_int
+    L.ora_backward.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int]
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- primitives
+def tri_cast(tri, o, d, clip=False):
+    t = ctypes.c_float(0.0)
+    p = np.zeros(3, np.float32)
+    hit = lib().ora_tri_cast(_f32(tri), _f32(o), _f32(d), int(clip), ctypes.byref(t), p)
+    return bool(hit), np.float32(t.value), p
+
+
+def tri_contains(tri, p):
+    return bool(lib().ora_tri_contains(_f32(tri), _f32(p)))
+
+
+def aabb_cast_preinv(box, o, inv):
+    return np.float32(lib().ora_aabb_cast_preinv(_f32(box), _f32(o), _f32(inv)))
+
+
+def aabb_intersects_tri(box, tri):
+    return bool(lib().ora_aabb_intersects_tri(_f32(box), _f32(tri)))
+
+
+def intersection_bounds(box, tri):
+    out = np.zeros(6, np.float32)
+    lib().ora_intersection_bounds(_f32(box), _f32(tri), out)
+    return out
+
+
+def intersection_center(box, tri):
+    out = np.zeros(3, np.float32)
+    lib().ora_intersection_center(_f32(box), _f32(tri), out)
+    return out
+
+
+def clip_tri(box, tri):
+    out = np.zeros(27 * 9, np.float32)
+    n = lib().ora_clip_tri(_f32(box), _f32(tri), out)
+    return out[: 9 * min(n, 27)].reshape(-1, 9), n
+
+
+def tri_get_masses(tri, p):
+    out = np.zeros(3, np.float32)
+    lib().ora_tri_get_masses(_f32(tri), _f32(p), out)
+    return out
+
+
+def tri_mass_center(tri, m):
+    out = np.zeros(3, np.float32)
+    lib().ora_tri_mass_center_w(_f32(tri), _f32(m), out)
+    return out
+
+
+# ---------------------------------------------------------------- octree
+class Octree:
+    """Oracle octree: Octree<Renderable<BakedTriFace>>::push(all) + build()."""
+
+    def __init__(self, tris, split_threshold=16, max_depth=12):
+        self.tris = _f32(tris).reshape(-1, 9)
+        self.h = lib().ora_octree_build(self.tris, int(self.tris.shape[0]), int(split_threshold), int(max_depth))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_octree_free(self.h)
+            self.h = None
+
+    @property
+    def node_count(self):
+        return lib().ora_octree_node_count(self.h)
+
+    def export(self):
+        n = self.node_count
+        tot = lib().ora_octree_list_total(self.h)
+        bounds = np.zeros(n * 6, np.float32)
+        children = np.zeros(n, np.int32)
+        off = np.zeros(n + 1, np.int32)
+        lst = np.zeros(max(tot, 1), np.int32)
+        lib().ora_octree_export(self.h, bounds, children, off, lst)
+        return bounds.reshape(n, 6), children, off, lst[:tot]
+
+    def cast(self, rays, clip=False, exclude=None, counters=False, points=False, nthreads=0):
+        rays = _f32(rays).reshape(-1, 6)
+        n = rays.shape[0]
+        ids = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.int32)
+        pts = np.zeros((n, 3), np.float32) if points else None
+        nb = np.zeros(n, np.int32) if counters else None
+        nr = np.zeros(n, np.int32) if counters else None
+        nt = np.zeros(n, np.int32) if counters else None
+        lib().ora_octree_cast(self.h, rays, n, int(clip), _ptr(ex), ids, t, _ptr(pts), _ptr(nb), _ptr(nr),
+                              _ptr(nt), int(nthreads))
+        out = {"id": ids, "t": t}
+        if points:
+            out["point"] = pts
+        if counters:
+            out.update(n_box=nb, n_leafref=nr, n_tri=nt)
+        return out
+
+
+def brute_cast(tris, rays, clip=False, exclude=None, nthreads=0):
+    tris = _f32(tris).reshape(-1, 9)
+    rays = _f32(rays).reshape(-1, 6)
+    n = rays.shape[0]
+    ids = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    second = np.zeros(n, np.float32)
+    ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.int32)
+    lib().ora_brute_cast(tris, int(tris.shape[0]), rays, n, int(clip), _ptr(ex), ids, t, _ptr(second), int(nthreads))
+    return {"id": ids, "t": t, "second": second}
