@@ -1,29 +1,95 @@
-// Batch closest-hit ray cast: one ray per lane, wave64, LDS traversal stack.
+// Batch closest-hit ray cast.
+//
+// Each 256-lane block owns a contiguous chunk of rays.  Lanes whose ray has
+// finished take the next ray of the chunk: the wave ballots its idle lanes,
+// one lane reserves that many rays with a single LDS atomic, and each idle lane
+// takes slot base + (number of idle lanes below it) (wave64 ballot + mbcnt
+// prefix).  Every loop iteration a lane either advances to its next non-empty
+// leaf or tests the triangles of the leaf it holds, so the triangle tests of a
+// wave run together instead of interleaving with node expansion.
 #include "dr_common.h"
 
 namespace dr {
 
 constexpr int kCastBlock = 256;
+constexpr int kCastChunk = 2048;      // rays per block
 
-// Traversal stack in LDS, [level][lane]: kMaxTreeDepth levels x 256 lanes x 4 B = 24 KiB.
-__global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__restrict__ rays, int n, int clip,
-                                                      const int *__restrict__ exclude, int *__restrict__ out_id,
-                                                      float *__restrict__ out_t, float *__restrict__ out_p) {
-  __shared__ uint32_t stk[kMaxTreeDepth * kCastBlock];
-  const int r = blockIdx.x * kCastBlock + threadIdx.x;
-  if (r >= n) return;
-  const float2 *R = (const float2 *)(rays + 6 * (size_t)r);
-  const float2 q0 = R[0], q1 = R[1], q2 = R[2];
-  const V3 o = v3(q0.x, q0.y, q1.x), d = v3(q1.y, q2.x, q2.y);
-  const int ex = exclude ? exclude[r] : -1;
-  Hit h;
-  const bool hit = tree_cast(T, o, d, clip != 0, ex, stk + threadIdx.x, kCastBlock, h);
+__device__ __forceinline__ void write_result(int r, bool hit, const Hit &h, int *__restrict__ out_id,
+                                             float *__restrict__ out_t, float *__restrict__ out_p) {
   out_id[r] = hit ? h.id : -1;
   out_t[r] = hit ? h.t : FLT_MAX;
   if (out_p) {
     out_p[3 * (size_t)r + 0] = hit ? h.p.x : 0.0f;
     out_p[3 * (size_t)r + 1] = hit ? h.p.y : 0.0f;
     out_p[3 * (size_t)r + 2] = hit ? h.p.z : 0.0f;
+  }
+}
+
+// Dynamic LDS: `levels` stack words per lane, [level][lane].
+__global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__restrict__ rays, int n, int clip,
+                                                      const int *__restrict__ exclude, int *__restrict__ out_id,
+                                                      float *__restrict__ out_t, float *__restrict__ out_p) {
+  extern __shared__ uint32_t stk_base[];
+  __shared__ int next_ray;
+  uint32_t *stk = stk_base + threadIdx.x;
+  const int chunk_end = min(n, (int)(blockIdx.x + 1) * kCastChunk);
+  if (threadIdx.x == 0) next_ray = blockIdx.x * kCastChunk;
+  __syncthreads();
+
+  const int lane = (int)__lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int r = -1;
+  bool exhausted = false;
+  TRay ray;
+  int ex = -1, sp = -1;
+  bool leaf = false;
+  int lg = -1, lk = 0, llink = 0, lcnt = 0;
+  Hit h;
+  h.id = -1; h.t = FLT_MAX; h.p = v3(0.f, 0.f, 0.f);
+
+  for (;;) {
+    // ---- refill idle lanes from the block's chunk
+    const bool want = (r < 0) && !exhausted;
+    const uint64_t wmask = __ballot(want);
+    if (wmask) {
+      const int leader = __ffsll((unsigned long long)wmask) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&next_ray, __popcll(wmask));
+      base = __shfl(base, leader);
+      if (want) {
+        const int slot = base + __popcll(wmask & below);
+        if (slot < chunk_end) {
+          r = slot;
+          const float2 *R = (const float2 *)(rays + 6 * (size_t)r);
+          const float2 q0 = R[0], q1 = R[1], q2 = R[2];
+          ray = make_tray(v3(q0.x, q0.y, q1.x), v3(q1.y, q2.x, q2.y));
+          ex = exclude ? exclude[r] : -1;
+          const int st = tree_begin(T, ray, stk, sp);
+          if (st == 0) { write_result(r, false, h, out_id, out_t, out_p); r = -1; }
+          else if (st == 2) { leaf = true; lg = -1; llink = T.root_start; lcnt = T.root_count; }
+        } else {
+          exhausted = true;
+        }
+      }
+    }
+    if (__ballot(r >= 0) == 0) {
+      if (__ballot(!exhausted) == 0) break;
+      continue;
+    }
+    // ---- advance to the next non-empty leaf in DFS order
+    if (r >= 0 && !leaf) {
+      if (next_leaf(T.groups, ray, stk, kCastBlock, sp, lg, lk, llink, lcnt)) leaf = true;
+      else { write_result(r, false, h, out_id, out_t, out_p); r = -1; }
+    }
+    // ---- castInLeaf
+    if (leaf) {
+      leaf = false;
+      Box lb;
+      if (lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
+      else lb = child_box(T.groups, lg, lk);
+      const bool hit = leaf_cast(T.recs, llink, lcnt, lb, ray.o, ray.d, clip != 0, ex, h);
+      if (hit || lg < 0) { write_result(r, hit, h, out_id, out_t, out_p); r = -1; }
+    }
   }
 }
 
@@ -34,8 +100,10 @@ extern "C" int dr_cast(dr_scene s, const float *rays, int n, unsigned flags, con
   if (!s || n < 0 || (n > 0 && (!rays || !out_id || !out_t))) return dr::fail(DR_ERR_ARG, "dr_cast: bad arguments");
   if (s->max_depth > dr::kMaxTreeDepth) return dr::fail(DR_ERR_ARG, "dr_cast: octree deeper than the traversal stack");
   if (n == 0) return DR_OK;
-  const int blocks = (n + dr::kCastBlock - 1) / dr::kCastBlock;
-  hipLaunchKernelGGL(dr::k_cast, dim3(blocks), dim3(dr::kCastBlock), 0, (hipStream_t)stream, s->tree, rays, n,
+  const int levels = s->max_depth > 0 ? s->max_depth : 1;
+  const size_t lds = (size_t)levels * dr::kCastBlock * sizeof(uint32_t);
+  const int blocks = (n + dr::kCastChunk - 1) / dr::kCastChunk;
+  hipLaunchKernelGGL(dr::k_cast, dim3(blocks), dim3(dr::kCastBlock), lds, (hipStream_t)stream, s->tree, rays, n,
                      (int)(flags & DR_CAST_CLIP_BACKFACES), (const int *)exclude, (int *)out_id, out_t, out_point);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;

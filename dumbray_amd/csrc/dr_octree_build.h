@@ -22,18 +22,25 @@ struct OctreeHost {
 int build_octree(const float *tris9, int ntris, int split_threshold, int max_depth, int nthreads, OctreeHost &out);
 
 // ---------------------------------------------------------------- device layout
-// A "group" is the 8 contiguous children of one interior node, stored as 64
-// 32-bit words (256 B, one aligned pair of 128-B lines):
-//   [0..7]   min.x of child 0..7     [24..31] max.x
-//   [8..15]  min.y                   [32..39] max.y
-//   [16..23] min.z                   [40..47] max.z
-//   [48..55] link: interior -> child group index; leaf -> first triangle record
-//   [56..63] count: -1 interior, 0 empty leaf (or only degenerate triangles), >0 leaf records
+// A "group" is the 8 contiguous children of one interior node, stored as 128
+// 32-bit words (512 B, four aligned 128-B lines):
+//   [0..47]    reference boxes: min.x[8] min.y[8] min.z[8] max.x[8] max.y[8] max.z[8]
+//   [48..95]   subtree leaf bounds, same order: the union of the boxes of the
+//              non-empty leaves below the child (inverted +inf/-inf box if none)
+//   [96..103]  link: interior -> child group index; leaf -> first triangle record
+//   [104..111] count: -1 interior, 0 empty leaf (or only degenerate triangles), >0 records
+//   [112..127] unused
 // Child k uses the reference's numbering (bit0 = z-high, bit1 = y-high, bit2 = x-high).
-constexpr int kGroupWords = 64;
-// A triangle record is 16 words (64 B): a.xyz b.xyz c.xyz n.xyz thr0 thr1 thr2 id,
+// The subtree leaf bounds only prune work that cannot change the result: a
+// leaf accepts a hit only if the hit point o + d*t (t >= 0) lies in its box,
+// and that point never leaves the closed octant of o that d points into.
+constexpr int kGroupWords = 128;
+constexpr int kGroupF4 = kGroupWords / 4;
+// A triangle record is 16 words (64 B):
+//   a.x a.y a.z n.x | n.y n.z b.x b.y | b.z c.x c.y c.z | thr0 thr1 thr2 id
 // n = (b-a) x (c-a) and thr* = the ray-independent right-hand sides of the
-// containsVertex test.  Degenerate triangles (always a miss) get no record.
+// containsVertex test; the plane test reads only the first 32 B.
+// Degenerate triangles (always a miss) get no record.
 constexpr int kRecWords = 16;
 
 struct OctreeFlat {

@@ -404,8 +404,8 @@ namespace dr {
 
 static void put_rec(std::vector<float> &recs, const Tri &t, int32_t id) {
   const TriPre p = tri_pre(t);
-  float w[kRecWords] = {t.a.x, t.a.y, t.a.z, t.b.x, t.b.y, t.b.z, t.c.x, t.c.y, t.c.z,
-                        p.n.x, p.n.y, p.n.z, p.thr0, p.thr1, p.thr2, 0.0f};
+  float w[kRecWords] = {t.a.x, t.a.y, t.a.z, p.n.x, p.n.y, p.n.z, t.b.x, t.b.y,
+                        t.b.z, t.c.x, t.c.y, t.c.z, p.thr0, p.thr1, p.thr2, 0.0f};
   memcpy(&w[15], &id, 4);
   recs.insert(recs.end(), w, w + kRecWords);
 }
@@ -416,37 +416,63 @@ int flatten_octree(const OctreeHost &t, OctreeFlat &out) {
   if (nn < 1) return -1;
   out.root = t.bounds[0];
   out.max_depth = t.max_depth;
+  auto nondegenerate = [&](int32_t id) {
+    const Tri &tr = t.tris[id];
+    return !(eq(tr.a, tr.b) || eq(tr.b, tr.c) || eq(tr.c, tr.a));    // degenerate never hits (Triangle.impl.h:292)
+  };
   auto leaf_records = [&](int node, int32_t &start, int32_t &count) {
     start = (int32_t)(out.recs.size() / kRecWords);
     count = 0;
     for (int k = t.list_off[node]; k < t.list_off[node + 1]; k++) {
-      const int32_t id = t.list[k];
-      const Tri &tr = t.tris[id];
-      if (eq(tr.a, tr.b) || eq(tr.b, tr.c) || eq(tr.c, tr.a)) continue;   // never hits (Triangle.impl.h:292)
-      put_rec(out.recs, tr, id);
+      if (!nondegenerate(t.list[k])) continue;
+      put_rec(out.recs, t.tris[t.list[k]], t.list[k]);
       count++;
     }
   };
   if (t.children[0] < 0) {
     out.root_is_leaf = 1;
     leaf_records(0, out.root_start, out.root_count);
-  } else {
-    out.ngroups = (nn - 1) / 8;
-    if ((nn - 1) % 8 != 0 || out.ngroups >= (1 << 24)) return -1;
-    out.groups.assign((size_t)out.ngroups * kGroupWords, 0.0f);
-    for (int g = 0; g < out.ngroups; g++) {
-      float *G = &out.groups[(size_t)g * kGroupWords];
-      for (int k = 0; k < 8; k++) {
-        const int node = 1 + 8 * g + k;
-        const Box &b = t.bounds[node];
-        G[0 + k] = b.s.x; G[8 + k] = b.s.y; G[16 + k] = b.s.z;
-        G[24 + k] = b.e.x; G[32 + k] = b.e.y; G[40 + k] = b.e.z;
-        int32_t link, count;
-        if (t.children[node] >= 0) { link = (t.children[node] - 1) / 8; count = -1; }
-        else leaf_records(node, link, count);
-        memcpy(&G[48 + k], &link, 4);
-        memcpy(&G[56 + k], &count, 4);
+    out.nrec = (int)(out.recs.size() / kRecWords);
+    return 0;
+  }
+  out.ngroups = (nn - 1) / 8;
+  if ((nn - 1) % 8 != 0 || out.ngroups >= (1 << 24)) return -1;
+  // subtree leaf bounds, children before parents (breadth-first order reversed)
+  const float inf = __builtin_inff();
+  std::vector<Box> slb(nn);
+  for (int i = nn - 1; i >= 0; i--) {
+    Box b;
+    b.s = v3(inf, inf, inf);
+    b.e = v3(-inf, -inf, -inf);
+    if (t.children[i] < 0) {
+      bool any = false;
+      for (int k = t.list_off[i]; k < t.list_off[i + 1] && !any; k++) any = nondegenerate(t.list[k]);
+      if (any) b = t.bounds[i];
+    } else {
+      for (int c = t.children[i]; c < t.children[i] + 8; c++) {
+        const Box &q = slb[c];
+        b.s = v3(std::min(b.s.x, q.s.x), std::min(b.s.y, q.s.y), std::min(b.s.z, q.s.z));
+        b.e = v3(std::max(b.e.x, q.e.x), std::max(b.e.y, q.e.y), std::max(b.e.z, q.e.z));
       }
+    }
+    slb[i] = b;
+  }
+  out.groups.assign((size_t)out.ngroups * kGroupWords, 0.0f);
+  for (int g = 0; g < out.ngroups; g++) {
+    float *G = &out.groups[(size_t)g * kGroupWords];
+    for (int k = 0; k < 8; k++) {
+      const int node = 1 + 8 * g + k;
+      const Box &b = t.bounds[node];
+      const Box &l = slb[node];
+      G[0 + k] = b.s.x; G[8 + k] = b.s.y; G[16 + k] = b.s.z;
+      G[24 + k] = b.e.x; G[32 + k] = b.e.y; G[40 + k] = b.e.z;
+      G[48 + k] = l.s.x; G[56 + k] = l.s.y; G[64 + k] = l.s.z;
+      G[72 + k] = l.e.x; G[80 + k] = l.e.y; G[88 + k] = l.e.z;
+      int32_t link, count;
+      if (t.children[node] >= 0) { link = (t.children[node] - 1) / 8; count = -1; }
+      else leaf_records(node, link, count);
+      memcpy(&G[96 + k], &link, 4);
+      memcpy(&G[104 + k], &count, 4);
     }
   }
   out.nrec = (int)(out.recs.size() / kRecWords);

@@ -7,14 +7,18 @@
 // cast, and inside a leaf the first triangle with the strictly smallest t wins.
 // The box tests of all 8 children are evaluated together when a node is
 // expanded (they are pure, so this only changes when, not whether, they run).
+// Children whose subtree leaves all lie outside the ray's forward octant are
+// dropped: no hit point o + d*t with t >= 0 can be inside their boxes, so the
+// reference would visit them and reject every hit.
 #pragma once
 #include "dr_geom.h"
+#include "dr_octree_build.h"
 
 namespace dr {
 
 struct DevTree {
-  const float4 *__restrict__ groups;   // 16 float4 per group (kGroupWords)
-  const float4 *__restrict__ recs;     // 4 float4 per triangle record (kRecWords)
+  const float4 *__restrict__ groups;   // kGroupF4 float4 per group
+  const float4 *__restrict__ recs;     // 4 float4 per triangle record
   float rsx, rsy, rsz, rex, rey, rez;  // root box
   int root_is_leaf, root_start, root_count;
   int pad;
@@ -25,6 +29,23 @@ struct Hit {
   float t;
   V3 p;
 };
+
+// A ray prepared for traversal.
+struct TRay {
+  V3 o, d, inv;
+  int canon;     // canonicalOrder (Octree.impl.cuh:181-183)
+};
+__device__ __forceinline__ TRay make_tray(V3 o, V3 d) {
+  TRay r;
+  r.o = o;
+  r.d = d;
+  r.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  int c = (d.z < 0) ? 1 : 0;
+  if (d.y < 0) c += 2;
+  if (d.x < 0) c += 4;
+  r.canon = c;
+  return r;
+}
 
 // Triangle::containsVertex minus the degenerate test (degenerate triangles have no record)
 __device__ __forceinline__ bool inside(V3 p, V3 a, V3 b, V3 c, float thr0, float thr1, float thr2) {
@@ -43,66 +64,74 @@ __device__ __forceinline__ bool leaf_cast(const float4 *__restrict__ R, int star
   V3 bp = v3(0.f, 0.f, 0.f);
   int bid = -1;
   for (int k = start; k < start + count; ++k) {
-    const float4 r0 = R[4 * k + 0], r1 = R[4 * k + 1], r2 = R[4 * k + 2];
-    const V3 a = v3(r0.x, r0.y, r0.z), n = v3(r2.y, r2.z, r2.w);
+    const float4 r0 = R[4 * k + 0], r1 = R[4 * k + 1];
+    const V3 a = v3(r0.x, r0.y, r0.z), n = v3(r0.w, r1.x, r1.y);
     const float dp = dot(a - o, n);
     if (clip && (dp < -kVecEps)) continue;
     const float dd = dot(d, n);
     if (dp * dd <= 0) continue;
     const float t = dp / dd;
     if (!(t < best)) continue;                       // could not replace the current best
-    const float4 r3 = R[4 * k + 3];
+    const float4 r2 = R[4 * k + 2], r3 = R[4 * k + 3];
     const int id = __float_as_int(r3.w);
     if (id == exclude) continue;
     const V3 p = o + d * t;
-    const V3 b = v3(r0.w, r1.x, r1.y), c = v3(r1.z, r1.w, r2.x);
+    const V3 b = v3(r1.z, r1.w, r2.x), c = v3(r2.y, r2.z, r2.w);
     if (inside(p, a, b, c, r3.x, r3.y, r3.z)) { best = t; bp = p; bid = id; }
   }
   if (best != FLT_MAX && contains(lb, bp)) { h.id = bid; h.t = best; h.p = bp; return true; }
   return false;
 }
 
-// 8 child box tests of group g -> mask in visiting order (bit i = child canonical ^ i),
-// empty leaves removed (castInLeaf on them returns false).
-__device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int g, V3 o, V3 inv, int canon) {
-  const float4 *q = G + 16 * (size_t)g;
-  float sx[8], sy[8], sz[8], ex[8], ey[8], ez[8];
-  int cnt[8];
-  *(float4 *)&sx[0] = q[0];  *(float4 *)&sx[4] = q[1];
-  *(float4 *)&sy[0] = q[2];  *(float4 *)&sy[4] = q[3];
-  *(float4 *)&sz[0] = q[4];  *(float4 *)&sz[4] = q[5];
-  *(float4 *)&ex[0] = q[6];  *(float4 *)&ex[4] = q[7];
-  *(float4 *)&ey[0] = q[8];  *(float4 *)&ey[4] = q[9];
-  *(float4 *)&ez[0] = q[10]; *(float4 *)&ez[4] = q[11];
-  *(float4 *)&cnt[0] = q[14]; *(float4 *)&cnt[4] = q[15];
+// Box of child k of group g (the leaf-acceptance box).
+__device__ __forceinline__ Box child_box(const float4 *__restrict__ G, int g, int k) {
+  const float *f = (const float *)(G + kGroupF4 * (size_t)g);
+  Box b;
+  b.s = v3(f[k], f[8 + k], f[16 + k]);
+  b.e = v3(f[24 + k], f[32 + k], f[40 + k]);
+  return b;
+}
+
+// 8 child tests of group g -> mask in visiting order (bit i = child canon ^ i):
+// reference slab test on the child's box, child has a non-empty leaf below it, and
+// that leaf's box can be reached by the forward ray (octant test on subtree bounds).
+__device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int g, const TRay &r) {
+  const float4 *q = G + kGroupF4 * (size_t)g;
   uint32_t m = 0;
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    m |= ((cnt[k] != 0) && slab_hit(sx[k], sy[k], sz[k], ex[k], ey[k], ez[k], o, inv)) ? (1u << k) : 0u;
-  if (canon & 1) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
-  if (canon & 2) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
-  if (canon & 4) m = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
+  for (int h = 0; h < 2; ++h) {       // two halves of 4 children keep register use low
+    const float4 sx = q[0 + h], sy = q[2 + h], sz = q[4 + h], ex = q[6 + h], ey = q[8 + h], ez = q[10 + h];
+    const float4 lsx = q[12 + h], lsy = q[14 + h], lsz = q[16 + h], lex = q[18 + h], ley = q[20 + h],
+                 lez = q[22 + h];
+    const float a_sx[4] = {sx.x, sx.y, sx.z, sx.w}, a_sy[4] = {sy.x, sy.y, sy.z, sy.w},
+                a_sz[4] = {sz.x, sz.y, sz.z, sz.w}, a_ex[4] = {ex.x, ex.y, ex.z, ex.w},
+                a_ey[4] = {ey.x, ey.y, ey.z, ey.w}, a_ez[4] = {ez.x, ez.y, ez.z, ez.w};
+    const float b_sx[4] = {lsx.x, lsx.y, lsx.z, lsx.w}, b_sy[4] = {lsy.x, lsy.y, lsy.z, lsy.w},
+                b_sz[4] = {lsz.x, lsz.y, lsz.z, lsz.w}, b_ex[4] = {lex.x, lex.y, lex.z, lex.w},
+                b_ey[4] = {ley.x, ley.y, ley.z, ley.w}, b_ez[4] = {lez.x, lez.y, lez.z, lez.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      // forward octant of o: coordinate k of o + d*t (t >= 0) is >= o.k if d.k >= 0, <= o.k if d.k <= 0
+      const bool behind = (r.d.x >= 0 && b_ex[j] < r.o.x) || (r.d.x <= 0 && b_sx[j] > r.o.x) ||
+                          (r.d.y >= 0 && b_ey[j] < r.o.y) || (r.d.y <= 0 && b_sy[j] > r.o.y) ||
+                          (r.d.z >= 0 && b_ez[j] < r.o.z) || (r.d.z <= 0 && b_sz[j] > r.o.z);
+      const bool hit = !behind && slab_hit(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv);
+      m |= hit ? (1u << (4 * h + j)) : 0u;
+    }
+  }
+  const int c = r.canon;
+  if (c & 1) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+  if (c & 2) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+  if (c & 4) m = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
   return m;
 }
 
-// Octree::cast.  stk points at this lane's first stack word; consecutive levels are
-// `stride` words apart (LDS, [level][lane] so a wave touches 64 consecutive words).
-__device__ __forceinline__ bool tree_cast(const DevTree &T, V3 o, V3 d, bool clip, int exclude, uint32_t *stk,
-                                          int stride, Hit &h) {
-  if (T.root_is_leaf) {
-    Box rb; rb.s = v3(T.rsx, T.rsy, T.rsz); rb.e = v3(T.rex, T.rey, T.rez);
-    return leaf_cast(T.recs, T.root_start, T.root_count, rb, o, d, clip, exclude, h);
-  }
-  const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  if (!slab_hit(T.rsx, T.rsy, T.rsz, T.rex, T.rey, T.rez, o, inv)) return false;
-  int canon = (d.z < 0) ? 1 : 0;
-  if (d.y < 0) canon += 2;
-  if (d.x < 0) canon += 4;
-  const float4 *__restrict__ G = T.groups;
-  const int *__restrict__ Gi = (const int *)G;
-  const float *__restrict__ Gf = (const float *)G;
-  int sp = 0;
-  stk[0] = group_mask(G, 0, o, inv, canon);           // (group << 8) | mask, group 0 = root's children
+// Pops the next child to visit.  Returns 1 with (g, k, link, cnt) set when a
+// non-empty leaf was reached, pushes interior children, returns 0 when the
+// stack is exhausted (miss).
+__device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, const TRay &r, uint32_t *stk, int stride,
+                                         int &sp, int &g_out, int &k_out, int &link_out, int &cnt_out) {
+  const int *Gi = (const int *)G;
   while (sp >= 0) {
     const uint32_t f = stk[sp * stride];
     const uint32_t m = f & 0xffu;
@@ -110,20 +139,45 @@ __device__ __forceinline__ bool tree_cast(const DevTree &T, V3 o, V3 d, bool cli
     const int i = __builtin_ctz(m);
     stk[sp * stride] = f & ~(1u << i);
     const int g = (int)(f >> 8);
-    const int k = canon ^ i;
-    const size_t base = 64 * (size_t)g;
-    const int link = Gi[base + 48 + k];
-    const int cnt = Gi[base + 56 + k];
+    const int k = r.canon ^ i;
+    const size_t base = kGroupWords * (size_t)g;
+    const int link = Gi[base + 96 + k];
+    const int cnt = Gi[base + 104 + k];
     if (cnt < 0) {
       ++sp;
-      stk[sp * stride] = ((uint32_t)link << 8) | group_mask(G, link, o, inv, canon);
+      stk[sp * stride] = ((uint32_t)link << 8) | group_mask(G, link, r);
     } else {
-      Box lb;
-      lb.s = v3(Gf[base + k], Gf[base + 8 + k], Gf[base + 16 + k]);
-      lb.e = v3(Gf[base + 24 + k], Gf[base + 32 + k], Gf[base + 40 + k]);
-      if (leaf_cast(T.recs, link, cnt, lb, o, d, clip, exclude, h)) return true;
+      g_out = g; k_out = k; link_out = link; cnt_out = cnt;
+      return 1;
     }
   }
+  return 0;
+}
+
+// Initial traversal state.  Returns 0 = miss, 1 = stack ready, 2 = root is a leaf.
+__device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint32_t *stk, int &sp) {
+  if (T.root_is_leaf) return 2;
+  if (!slab_hit(T.rsx, T.rsy, T.rsz, T.rex, T.rey, T.rez, r.o, r.inv)) return 0;
+  sp = 0;
+  stk[0] = group_mask(T.groups, 0, r);               // group 0 = the root's children
+  return 1;
+}
+
+// Octree::cast for one ray.  stk points at this lane's first stack word;
+// consecutive levels are `stride` words apart (LDS, [level][lane]).
+__device__ __forceinline__ bool tree_cast(const DevTree &T, V3 o, V3 d, bool clip, int exclude, uint32_t *stk,
+                                          int stride, Hit &h) {
+  const TRay r = make_tray(o, d);
+  int sp = -1;
+  const int st = tree_begin(T, r, stk, sp);
+  if (st == 0) return false;
+  if (st == 2) {
+    Box rb; rb.s = v3(T.rsx, T.rsy, T.rsz); rb.e = v3(T.rex, T.rey, T.rez);
+    return leaf_cast(T.recs, T.root_start, T.root_count, rb, o, d, clip, exclude, h);
+  }
+  int g, k, link, cnt;
+  while (next_leaf(T.groups, r, stk, stride, sp, g, k, link, cnt))
+    if (leaf_cast(T.recs, link, cnt, child_box(T.groups, g, k), o, d, clip, exclude, h)) return true;
   return false;
 }
 

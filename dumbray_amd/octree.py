@@ -135,7 +135,9 @@ class DeviceScene:
         pts = torch.empty((n, 3), dtype=torch.float32, device=rays.device) if points else None
         ex = None
         if exclude is not None:
-            ex = exclude.contiguous().to(device=rays.device, dtype=torch.int32)
+            ex = exclude.contiguous().to(device=rays.device, dtype=torch.int32).view(-1)
+            if ex.numel() != n:
+                raise DumbRayError("cast: exclude must hold one triangle id per ray")
         s = stream if stream is not None else torch.cuda.current_stream(rays.device)
         rc = lib().dr_cast(self._h, _vp(rays.data_ptr()), int(n), DR_CAST_CLIP_BACKFACES if clip_backfaces else 0,
                            _vp(ex.data_ptr()) if ex is not None else None, _vp(ids.data_ptr()), _vp(t.data_ptr()),
