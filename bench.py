@@ -25,11 +25,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0                      # MI355X HBM3E
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # lane-ops/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
-# algorithmic work per ray (reference algorithm, counted by the oracle on the same rays):
+# Work of the reference algorithm per ray (counted by the oracle on the same rays):
 # bytes = 24 ray in + 8 (id, t) out + 28 per box (6 fp32 bounds + link) + 4 per leaf-list entry
 #         + 36 per triangle test (9 fp32 vertices); ops = 6 + 24 per box + 155 per triangle test
 B_RAY, B_OUT, B_BOX, B_REF, B_TRI = 24, 8, 28, 4, 36
 OPS_RAY, OPS_BOX, OPS_TRI = 6, 24, 155
+# Work of the kernel's traversal per ray (counted by dr_cast_stats on the same rays): a group
+# expansion reads 384 B (8 boxes + 8 subtree bounds) and pops read 8 B (link, count); a leaf
+# reads its 24-B box; a plane test reads 32 B of the record, an inside test 32 B more.
+# ops: 6 (1/d) + 288 per expansion (8 x (24 slab + 12 octant)) + 25 per plane test
+#      + 140 per inside test (6 edge vectors, 3 dots, 3 sqrt, 3 IEEE div).
+K_RAY, K_OUT, K_GROUP, K_POP, K_LEAF, K_PLANE, K_INSIDE = 24, 8, 384, 8, 24, 32, 32
+KOPS_RAY, KOPS_GROUP, KOPS_PLANE, KOPS_INSIDE = 6, 288, 25, 140
 
 
 def log(msg):
@@ -61,6 +68,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (pmc calibration)")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the timed incoherent casts (+ calibration copy): for rocprofv3 runs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -94,7 +103,8 @@ def main():
 
     for _ in range(args.warmup):
         step(rB)
-        step(rA)
+        if not args.profile:
+            step(rA)
     torch.cuda.synchronize()
 
     def timed(r, k):
@@ -111,17 +121,29 @@ def main():
 
     kB_ms, wallB, outB = timed(rB, args.steps)
     log(f"batch B: {kB_ms:.3f} ms/launch ({len(incoherent) / kB_ms / 1e3:.1f} Mrays/s)")
+    if args.profile:
+        if args.calib_copy:
+            a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+            b = torch.empty_like(a)
+            b.copy_(a)
+            torch.cuda.synchronize()
+            log("calibration copy: 268435456 bytes read + written")
+        print(json.dumps({"profile": True, "kernel_ms": kB_ms}), flush=True)
+        return
+    st = scene.cast_stats(rB, clip_backfaces=False, stream=stream)
+    torch.cuda.synchronize()
+    cnt = st["counts"].double().mean(dim=0).cpu().numpy()
+    assert torch.equal(st["id"], outB["id"]), "dr_cast_stats disagrees with dr_cast"
+    ge, lv, pl, ins = (float(x) for x in cnt)
+    kernel_work = {"group_expansions": ge, "leaves": lv, "plane_tests": pl, "inside_tests": ins,
+                   "bytes": K_RAY + K_OUT + K_GROUP * ge + K_POP * (ge - 1 + lv) + K_LEAF * lv + K_PLANE * pl
+                   + K_INSIDE * ins,
+                   "ops": KOPS_RAY + KOPS_GROUP * ge + KOPS_PLANE * pl + KOPS_INSIDE * ins}
+    log(f"kernel work per ray: {kernel_work}")
     kA_ms, wallA, outA = timed(rA, args.steps)
     log(f"batch A: {kA_ms:.3f} ms/launch ({len(coherent) / kA_ms / 1e3:.1f} Mrays/s)")
     ms_per_step = wallB * 1e3 / args.steps
     value = len(incoherent) / (ms_per_step / 1e3) / 1e6
-
-    if args.calib_copy:
-        a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("calibration copy: 268435456 bytes read + written")
 
     # ---------------------------------------------------------------- oracle leg
     per_ray = None
@@ -143,34 +165,30 @@ def main():
                    "bytes": B_RAY + B_OUT + B_BOX * nb + B_REF * nr + B_TRI * nt,
                    "ops": OPS_RAY + OPS_BOX * nb + OPS_TRI * nt}
         log(f"oracle sample parity mismatches={parity}; per ray boxes={nb:.1f} refs={nr:.1f} tris={nt:.1f}")
-        # CPU baseline: oracle cast on a bounded sample sized to ~cpu_seconds
-        probe = incoherent[:20000]
-        t0 = time.perf_counter()
-        ot.cast(probe, nthreads=threads)
-        rate = len(probe) / (time.perf_counter() - t0)
-        ns = int(min(len(incoherent), max(20000, rate * args.cpu_seconds)))
-        sample = incoherent[:ns]
-        t0 = time.perf_counter()
-        ot.cast(sample, nthreads=threads)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": round(ns / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"first {ns} of the 4,194,304 incoherent rays, oracle Octree::cast, {threads} threads, "
-                         f"{cpu_s:.1f} s"}
+        # CPU baseline: the oracle's Octree::cast over the whole incoherent batch, repeated
+        # until ~cpu_seconds have passed (at least 3 runs), median rate
+        rates, spent = [], 0.0
+        while len(rates) < 3 or (spent < args.cpu_seconds and len(rates) < 50):
+            t0 = time.perf_counter()
+            ot.cast(incoherent, nthreads=threads)
+            dt = time.perf_counter() - t0
+            spent += dt
+            rates.append(len(incoherent) / dt / 1e6)
+        cpu = {"value": round(float(np.median(rates)), 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"all {len(incoherent)} incoherent rays through the oracle's Octree::cast on {threads} "
+                         f"threads, median of {len(rates)} runs ({spent:.1f} s); octree build excluded"}
         log(f"cpu baseline: {cpu['value']} Mrays/s on {threads} threads")
 
-    roofline = None
-    valu = None
-    if per_ray is not None:
-        algo_bytes = per_ray["bytes"] * len(incoherent)
-        achieved = algo_bytes / (kB_ms / 1e3) / 1e9
-        traffic = read_traffic(os.path.join(ROOT, "profiles", "cast_incoherent_hf2M_traffic.txt"))
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "dr::k_cast", "kernel_ms": round(kB_ms, 4),
-                    "algo_bytes_per_launch": int(algo_bytes)}
-        ops = per_ray["ops"] * len(incoherent)
-        valu = {"achieved": round(ops / (kB_ms / 1e3) / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                "unit": "T lane-ops/s", "frac": round(ops / (kB_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)}
+    algo_bytes = kernel_work["bytes"] * len(incoherent)
+    achieved = algo_bytes / (kB_ms / 1e3) / 1e9
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "cast_incoherent_hf2M_traffic.txt"))
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "dr::k_cast<false>", "kernel_ms": round(kB_ms, 4),
+                "algo_bytes_per_launch": int(algo_bytes)}
+    ops = kernel_work["ops"] * len(incoherent)
+    valu = {"achieved": round(ops / (kB_ms / 1e3) / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "T lane-ops/s", "frac": round(ops / (kB_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)}
 
     line = {
         "metric": "closest-hit Mrays/s & Mpix·spp/s at 1080p; adjoint/forward time ratio",
@@ -193,7 +211,8 @@ def main():
         "octree_build_s": round(build_s, 3),
         "roofline": roofline,
         "valu": valu,
-        "per_ray_work": per_ray,
+        "kernel_work_per_ray": kernel_work,
+        "reference_work_per_ray": per_ray,
         "oracle_sample_mismatches": parity,
         "cpu_baseline": cpu,
     }

@@ -25,10 +25,13 @@ __device__ __forceinline__ void write_result(int r, bool hit, const Hit &h, int 
   }
 }
 
-// Dynamic LDS: `levels` stack words per lane, [level][lane].
+// Dynamic LDS: `levels` stack words per lane, [level][lane].  kStats: also write
+// the per-ray work counters (4 x int32 per ray) to `stats`.
+template <bool kStats>
 __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__restrict__ rays, int n, int clip,
                                                       const int *__restrict__ exclude, int *__restrict__ out_id,
-                                                      float *__restrict__ out_t, float *__restrict__ out_p) {
+                                                      float *__restrict__ out_t, float *__restrict__ out_p,
+                                                      int *__restrict__ stats) {
   extern __shared__ uint32_t stk_base[];
   __shared__ int next_ray;
   uint32_t *stk = stk_base + threadIdx.x;
@@ -46,6 +49,17 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
   int lg = -1, lk = 0, llink = 0, lcnt = 0;
   Hit h;
   h.id = -1; h.t = FLT_MAX; h.p = v3(0.f, 0.f, 0.f);
+  CastCounters cnt_local = {0, 0, 0, 0};
+  CastCounters *cc = kStats ? &cnt_local : nullptr;
+  auto flush_stats = [&](int ray_index) {
+    if (kStats) {
+      stats[4 * (size_t)ray_index + 0] = cnt_local.expand;
+      stats[4 * (size_t)ray_index + 1] = cnt_local.leaves;
+      stats[4 * (size_t)ray_index + 2] = cnt_local.plane;
+      stats[4 * (size_t)ray_index + 3] = cnt_local.inside;
+      cnt_local = {0, 0, 0, 0};
+    }
+  };
 
   for (;;) {
     // ---- refill idle lanes from the block's chunk
@@ -64,8 +78,8 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
           const float2 q0 = R[0], q1 = R[1], q2 = R[2];
           ray = make_tray(v3(q0.x, q0.y, q1.x), v3(q1.y, q2.x, q2.y));
           ex = exclude ? exclude[r] : -1;
-          const int st = tree_begin(T, ray, stk, sp);
-          if (st == 0) { write_result(r, false, h, out_id, out_t, out_p); r = -1; }
+          const int st = tree_begin(T, ray, stk, sp, cc);
+          if (st == 0) { write_result(r, false, h, out_id, out_t, out_p); flush_stats(r); r = -1; }
           else if (st == 2) { leaf = true; lg = -1; llink = T.root_start; lcnt = T.root_count; }
         } else {
           exhausted = true;
@@ -78,8 +92,8 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
     }
     // ---- advance to the next non-empty leaf in DFS order
     if (r >= 0 && !leaf) {
-      if (next_leaf(T.groups, ray, stk, kCastBlock, sp, lg, lk, llink, lcnt)) leaf = true;
-      else { write_result(r, false, h, out_id, out_t, out_p); r = -1; }
+      if (next_leaf(T.groups, ray, stk, kCastBlock, sp, lg, lk, llink, lcnt, cc)) leaf = true;
+      else { write_result(r, false, h, out_id, out_t, out_p); flush_stats(r); r = -1; }
     }
     // ---- castInLeaf
     if (leaf) {
@@ -87,10 +101,28 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
       Box lb;
       if (lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
       else lb = child_box(T.groups, lg, lk);
-      const bool hit = leaf_cast(T.recs, llink, lcnt, lb, ray.o, ray.d, clip != 0, ex, h);
-      if (hit || lg < 0) { write_result(r, hit, h, out_id, out_t, out_p); r = -1; }
+      const bool hit = leaf_cast(T.recs, llink, lcnt, lb, ray.o, ray.d, clip != 0, ex, h, cc);
+      if (hit || lg < 0) { write_result(r, hit, h, out_id, out_t, out_p); flush_stats(r); r = -1; }
     }
   }
+}
+
+static int launch_cast(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude, int32_t *out_id,
+                       float *out_t, float *out_point, int32_t *stats, void *stream) {
+  if (s->max_depth > kMaxTreeDepth) return fail(DR_ERR_ARG, "dr_cast: octree deeper than the traversal stack");
+  if (n == 0) return DR_OK;
+  const int levels = s->max_depth > 0 ? s->max_depth : 1;
+  const size_t lds = (size_t)levels * kCastBlock * sizeof(uint32_t);
+  const int blocks = (n + kCastChunk - 1) / kCastChunk;
+  const int clip = (int)(flags & DR_CAST_CLIP_BACKFACES);
+  if (stats)
+    hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(kCastBlock), lds, (hipStream_t)stream, s->tree, rays, n, clip,
+                       (const int *)exclude, (int *)out_id, out_t, out_point, (int *)stats);
+  else
+    hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(kCastBlock), lds, (hipStream_t)stream, s->tree, rays, n,
+                       clip, (const int *)exclude, (int *)out_id, out_t, out_point, (int *)nullptr);
+  DR_HIP_CHECK(hipGetLastError());
+  return DR_OK;
 }
 
 }  // namespace dr
@@ -98,13 +130,12 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
 extern "C" int dr_cast(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude, int32_t *out_id,
                        float *out_t, float *out_point, void *stream) {
   if (!s || n < 0 || (n > 0 && (!rays || !out_id || !out_t))) return dr::fail(DR_ERR_ARG, "dr_cast: bad arguments");
-  if (s->max_depth > dr::kMaxTreeDepth) return dr::fail(DR_ERR_ARG, "dr_cast: octree deeper than the traversal stack");
-  if (n == 0) return DR_OK;
-  const int levels = s->max_depth > 0 ? s->max_depth : 1;
-  const size_t lds = (size_t)levels * dr::kCastBlock * sizeof(uint32_t);
-  const int blocks = (n + dr::kCastChunk - 1) / dr::kCastChunk;
-  hipLaunchKernelGGL(dr::k_cast, dim3(blocks), dim3(dr::kCastBlock), lds, (hipStream_t)stream, s->tree, rays, n,
-                     (int)(flags & DR_CAST_CLIP_BACKFACES), (const int *)exclude, (int *)out_id, out_t, out_point);
-  DR_HIP_CHECK(hipGetLastError());
-  return DR_OK;
+  return dr::launch_cast(s, rays, n, flags, exclude, out_id, out_t, out_point, nullptr, stream);
+}
+
+extern "C" int dr_cast_stats(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude,
+                             int32_t *out_id, float *out_t, int32_t *out_counts, void *stream) {
+  if (!s || n < 0 || (n > 0 && (!rays || !out_id || !out_t || !out_counts)))
+    return dr::fail(DR_ERR_ARG, "dr_cast_stats: bad arguments");
+  return dr::launch_cast(s, rays, n, flags, exclude, out_id, out_t, nullptr, out_counts, stream);
 }

@@ -30,6 +30,14 @@ struct Hit {
   V3 p;
 };
 
+// Work counters of one cast (measurement launches only; nullptr elsewhere).
+struct CastCounters {
+  int expand;    // groups whose 8 child tests ran (512-B group reads)
+  int leaves;    // leaves whose triangles were tested (+ leaf box and link/count reads)
+  int plane;     // triangle records whose plane test ran (first 32 B)
+  int inside;    // records that reached the inside test (second 32 B)
+};
+
 // A ray prepared for traversal.
 struct TRay {
   V3 o, d, inv;
@@ -59,7 +67,8 @@ __device__ __forceinline__ bool inside(V3 p, V3 a, V3 b, V3 c, float thr0, float
 // castInLeaf: closest accepted triangle among `count` records; accepted iff its hit
 // point lies inside the leaf box.  exclude = validateNotSameAsObject's object (or -1).
 __device__ __forceinline__ bool leaf_cast(const float4 *__restrict__ R, int start, int count, const Box &lb, V3 o,
-                                          V3 d, bool clip, int exclude, Hit &h) {
+                                          V3 d, bool clip, int exclude, Hit &h, CastCounters *cc = nullptr) {
+  if (cc) { cc->leaves++; cc->plane += count; }
   float best = FLT_MAX;
   V3 bp = v3(0.f, 0.f, 0.f);
   int bid = -1;
@@ -72,6 +81,7 @@ __device__ __forceinline__ bool leaf_cast(const float4 *__restrict__ R, int star
     if (dp * dd <= 0) continue;
     const float t = dp / dd;
     if (!(t < best)) continue;                       // could not replace the current best
+    if (cc) cc->inside++;
     const float4 r2 = R[4 * k + 2], r3 = R[4 * k + 3];
     const int id = __float_as_int(r3.w);
     if (id == exclude) continue;
@@ -130,7 +140,8 @@ __device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int
 // non-empty leaf was reached, pushes interior children, returns 0 when the
 // stack is exhausted (miss).
 __device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, const TRay &r, uint32_t *stk, int stride,
-                                         int &sp, int &g_out, int &k_out, int &link_out, int &cnt_out) {
+                                         int &sp, int &g_out, int &k_out, int &link_out, int &cnt_out,
+                                         CastCounters *cc = nullptr) {
   const int *Gi = (const int *)G;
   while (sp >= 0) {
     const uint32_t f = stk[sp * stride];
@@ -146,6 +157,7 @@ __device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, const TRa
     if (cnt < 0) {
       ++sp;
       stk[sp * stride] = ((uint32_t)link << 8) | group_mask(G, link, r);
+      if (cc) cc->expand++;
     } else {
       g_out = g; k_out = k; link_out = link; cnt_out = cnt;
       return 1;
@@ -155,11 +167,13 @@ __device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, const TRa
 }
 
 // Initial traversal state.  Returns 0 = miss, 1 = stack ready, 2 = root is a leaf.
-__device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint32_t *stk, int &sp) {
+__device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint32_t *stk, int &sp,
+                                          CastCounters *cc = nullptr) {
   if (T.root_is_leaf) return 2;
   if (!slab_hit(T.rsx, T.rsy, T.rsz, T.rex, T.rey, T.rez, r.o, r.inv)) return 0;
   sp = 0;
   stk[0] = group_mask(T.groups, 0, r);               // group 0 = the root's children
+  if (cc) cc->expand++;
   return 1;
 }
 
