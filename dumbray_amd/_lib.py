@@ -34,12 +34,10 @@ def _declare(L):
         "dr_cast": (c_int, [vp, vp, c_int, c_uint, vp, vp, vp, vp, vp]),
     }
     optional = {
-        "dr_scene_set_shading": (c_int, [vp, vp, vp, c_int, vp, c_int]),
-        "dr_render_blocks": (c_int, [vp, vp, c_int, c_int, c_int, c_int, vp, vp]),
-        "dr_render_pixels": (c_int, [vp, vp, vp, c_int, vp]),
-        "dr_backward": (c_int, [vp, vp, vp, vp, vp, vp]),
-        "dr_render_requests": (c_int, [vp, c_int, vp, vp, vp]),
         "dr_cast_stats": (c_int, [vp, vp, c_int, c_uint, vp, vp, vp, vp, vp]),
+        "dr_scene_set_materials": (c_int, [vp, vp, c_int]),
+        "dr_render_blocks": (c_int, [vp, vp, vp, c_int, c_int, vp]),
+        "dr_backward": (c_int, [vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in decl.items():
         f = getattr(L, name)

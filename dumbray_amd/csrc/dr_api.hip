@@ -112,7 +112,30 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
   T.root_start = flat.root_start;
   T.root_count = flat.root_count;
   s->max_depth = flat.max_depth;
+  s->ntris = (int)t->host.tris.size();
+  s->shade_host.resize(s->ntris);
+  for (int i = 0; i < s->ntris; i++) {
+    const dr::Tri &tr = t->host.tris[i];
+    const dr::V3 n = dr::cross(tr.b - tr.a, tr.c - tr.a);
+    s->shade_host[i] = make_float4(n.x, n.y, n.z, 0.0f);
+  }
   *out = s;
+  return DR_OK;
+}
+
+int dr_scene_set_materials(dr_scene s, const int32_t *mat_id, int ntris) {
+  if (!s || ntris != s->ntris || (ntris > 0 && !mat_id)) return dr::fail(DR_ERR_ARG, "dr_scene_set_materials: one id per triangle required");
+  int mx = -1;
+  for (int i = 0; i < ntris; i++) {
+    if (mat_id[i] < 0) return dr::fail(DR_ERR_ARG, "dr_scene_set_materials: negative material id");
+    memcpy(&s->shade_host[i].w, &mat_id[i], 4);
+    mx = mat_id[i] > mx ? mat_id[i] : mx;
+  }
+  s->max_mat = mx;
+  dr::dev_free(s->shade);
+  const int rc = dr::dev_upload(s->shade, s->shade_host.data(), sizeof(float4) * (size_t)ntris);
+  if (rc != DR_OK) return rc;
+  s->has_materials = true;
   return DR_OK;
 }
 
@@ -120,6 +143,7 @@ void dr_scene_destroy(dr_scene s) {
   if (!s) return;
   dr::dev_free(s->groups);
   dr::dev_free(s->recs);
+  dr::dev_free(s->shade);
   delete s;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include <vector>
 #include "../../include/dumbray.h"
 #include "dr_octree_build.h"
 #include "dr_traverse.h"
@@ -32,8 +33,13 @@ struct dr_octree_s {
 
 struct dr_scene_s {
   int device = 0;
-  dr::DevBuf groups, recs;
+  dr::DevBuf groups, recs, shade;
   dr::DevTree tree;
   int max_depth = 0;
-  int64_t bytes() const { return (int64_t)(groups.bytes + recs.bytes); }
+  int ntris = 0;
+  bool has_materials = false;
+  int max_mat = -1;                        // largest material id in use
+  // per triangle: n = (b-a) x (c-a) and the material id (bits of .w)
+  std::vector<float4> shade_host;
+  int64_t bytes() const { return (int64_t)(groups.bytes + recs.bytes + shade.bytes); }
 };

@@ -1,0 +1,383 @@
+// Path tracing: DumbRenderer::renderBlocksGPU and Renderer::backward on gfx950.
+//
+// One 256-lane workgroup renders one 8x16 framebuffer block.  The block's work
+// is a list of path jobs (pixel, sample); idle lanes take the next jobs with a
+// wave64 ballot + mbcnt prefix and one LDS atomic per wave, so lanes whose
+// path ended early are regenerated instead of idling.  Every loop iteration
+// each busy lane casts exactly one ray (camera/bounce ray or light photon) with
+// the shared octree traversal and then advances its path.  Finished sample
+// colours are staged in LDS and summed per pixel in the reference's subpixel
+// order before the block is written (set, or blended with 1/iteration).
+#include "dr_common.h"
+#include "dr_shade.h"
+
+namespace dr {
+
+constexpr int kRB = 256;               // lanes per workgroup
+constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per round
+constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
+
+// Per-lane path state.
+struct Path {
+  int job = -1;
+  uint32_t pix = 0, samp = 0;
+  int layer = 0, phase = 0, light = 0;  // phase 0: geometry ray, 1: light photon
+  V3 o, d;
+  int restrict_tri = -1;
+  float beta[3], L[3];
+  int h = -1, kind = 0, mid = 0;
+  V3 p, nu, du;
+  bool has_sample = false;
+  V3 sdir;
+  float sweight = 0.0f;
+  float contrib[3];                     // photon contribution if lit
+  float gl_k = 0.0f;                    // (adjoint) cos_i/pi * cos_l*A/d^2 of the pending photon
+};
+
+// Common job start: camera ray of sample `s` of local pixel `lp`.  Returns false
+// (and leaves the lane idle) for pixels outside the image.
+__device__ __forceinline__ bool start_path(const dr_render_params &P, int block, int lp, int s, uint32_t samp0,
+                                           Path &st) {
+  int x, y;
+  if (!block_pixel(P.width, P.height, block, lp, x, y)) return false;
+  st.pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
+  st.samp = samp0 + (uint32_t)s;
+  const int i = s / P.fsaa_y, j = s - (s / P.fsaa_y) * P.fsaa_y;
+  float u[4];
+  rng4(P, st.pix, st.samp, 0, 0, u);
+  camera_ray(P, x, y, i, j, u[0], u[1], st.o, st.d);
+  st.layer = 0;
+  st.phase = 0;
+  st.restrict_tri = -1;
+  for (int c = 0; c < 3; c++) { st.beta[c] = 1.0f; st.L[c] = 0.0f; }
+  return true;
+}
+
+// Sets up the next light photon of the current vertex (phase 1), or returns false.
+__device__ __forceinline__ bool next_photon(const dr_render_params &P, Path &st, float *cinfo = nullptr) {
+  for (int l = st.light + 1; l < P.nlights; l++) {
+    const dr_light Lt = P.lights[l];
+    V3 q, w;
+    float cos_i, gl;
+    if (!light_photon(P, Lt, l, st.pix, st.samp, st.layer, st.p, st.nu, q, w, cos_i, gl)) continue;
+    const dr_material M = P.materials[st.mid];
+    const float k = cos_i * kInvPi;
+    for (int c = 0; c < 3; c++) st.contrib[c] = st.beta[c] * ((M.albedo[c] * k) * (Lt.emission[c] * gl));
+    st.gl_k = k * gl;
+    st.light = l;
+    st.phase = 1;
+    st.o = q;
+    st.d = w;
+    st.restrict_tri = -1;
+    return true;
+  }
+  return false;
+}
+
+// Vertex setup after a geometry hit.
+__device__ __forceinline__ void setup_vertex(const dr_render_params &P, const float4 *__restrict__ shade, const Hit &hh,
+                                             Path &st) {
+  st.h = hh.id;
+  st.p = hh.p;
+  const float4 sh = shade[hh.id];
+  st.mid = __float_as_int(sh.w);
+  st.kind = P.materials[st.mid].kind;
+  st.du = normalized(st.d);
+  st.nu = facing_normal(v3(sh.x, sh.y, sh.z), st.du);
+  st.has_sample = next_sample(P, st.kind, st.pix, st.samp, st.layer, st.nu, st.du, st.sdir, st.sweight);
+  st.light = -1;
+}
+
+// Reflection factor A of the current vertex for its sampled direction.
+__device__ __forceinline__ void reflect_factor(const dr_render_params &P, const Path &st, float A[3]) {
+  const dr_material M = P.materials[st.mid];
+  if (st.kind == DR_METAL) {
+    const float f5 = schlick5(st.nu, st.sdir);
+    for (int c = 0; c < 3; c++) A[c] = M.albedo[c] + (1.0f - M.albedo[c]) * f5;
+  } else {
+    for (int c = 0; c < 3; c++) A[c] = M.albedo[c];
+  }
+}
+
+// ------------------------------------------------------------------ forward
+__global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
+                                                 float *__restrict__ frame, int start_block, int levels) {
+  extern __shared__ uint32_t lds[];
+  __shared__ float acc[kBlockPixels * 3];
+  __shared__ int next_job;
+  uint32_t *stk = lds + threadIdx.x;
+  float *samp_col = (float *)(lds + levels * kRB);
+  const int block = start_block + blockIdx.x;
+  const int spp = P.fsaa_x * P.fsaa_y;
+  const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
+  const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
+  const bool clip = P.cull_backfaces != 0;
+  const int lane = (int)__lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int t = threadIdx.x; t < kBlockPixels * 3; t += kRB) acc[t] = 0.0f;
+
+  for (int s_base = 0; s_base < spp; s_base += kRoundSamples) {
+    const int ns = min(kRoundSamples, spp - s_base);
+    const int njobs = kBlockPixels * ns;       // job j: local pixel j / ns, sample s_base + j % ns
+    if (threadIdx.x == 0) next_job = 0;
+    __syncthreads();
+    Path st;
+    bool exhausted = false;
+    auto finish = [&]() {
+      for (int c = 0; c < 3; c++) samp_col[3 * st.job + c] = st.L[c];
+      st.job = -1;
+    };
+    auto advance = [&]() {           // after the last photon of a vertex
+      if (!st.has_sample) { finish(); return; }
+      float A[3];
+      reflect_factor(P, st, A);
+      for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (A[c] * st.sweight);
+      st.o = st.p;
+      st.d = st.sdir;
+      st.restrict_tri = st.h;
+      st.layer++;
+      st.phase = 0;
+    };
+    for (;;) {
+      const bool want = (st.job < 0) && !exhausted;
+      const uint64_t wmask = __ballot(want);
+      if (wmask) {
+        const int leader = __ffsll((unsigned long long)wmask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&next_job, __popcll(wmask));
+        base = __shfl(base, leader);
+        if (want) {
+          const int slot = base + __popcll(wmask & below);
+          if (slot < njobs) {
+            st.job = slot;
+            const int lp = slot / ns, s = s_base + (slot - lp * ns);
+            if (!start_path(P, block, lp, s, samp0, st)) { for (int c = 0; c < 3; c++) st.L[c] = 0.0f; finish(); }
+          } else {
+            exhausted = true;
+          }
+        }
+      }
+      if (__ballot(st.job >= 0) == 0) {
+        if (__ballot(!exhausted) == 0) break;
+        continue;
+      }
+      if (st.job >= 0) {
+        Hit hh;
+        const bool hit = tree_cast(T, st.o, st.d, clip, st.restrict_tri, stk, kRB, hh);
+        if (st.phase == 0) {
+          if (!hit) finish();
+          else {
+            setup_vertex(P, shade, hh, st);
+            if (!(st.kind == DR_LAMBERT && next_photon(P, st))) advance();
+          }
+        } else {
+          const V3 e = hh.p - st.p;
+          const bool lit = hit && (hh.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
+          if (lit) for (int c = 0; c < 3; c++) st.L[c] += st.contrib[c];
+          if (!next_photon(P, st)) advance();
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kBlockPixels) {
+      for (int s = 0; s < ns; s++)
+        for (int c = 0; c < 3; c++) acc[3 * threadIdx.x + c] += samp_col[3 * (threadIdx.x * ns + s) + c] * spw;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kBlockPixels) {
+    int x, y;
+    if (block_pixel(P.width, P.height, block, threadIdx.x, x, y)) {
+      float4 *f = (float4 *)frame + (size_t)kBlockPixels * block + threadIdx.x;
+      const float4 col = make_float4(acc[3 * threadIdx.x], acc[3 * threadIdx.x + 1], acc[3 * threadIdx.x + 2], 1.0f);
+      const float amount = (P.iteration <= 1) ? 1.0f : (1.0f / ((float)P.iteration));
+      if (amount >= 1.0f) *f = col;
+      else {
+        const float4 old = *f;
+        const float ia = (1.0f - amount);
+        *f = make_float4((col.x * amount) + (old.x * ia), (col.y * amount) + (old.y * ia),
+                         (col.z * amount) + (old.z * ia), (col.w * amount) + (old.w * ia));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ adjoint
+// Vertex records of the lane's current path live in LDS ([vertex][word][lane]):
+// material id, reflection factor A (3), direct sum d (3) with L_k = A (d + s L_{k+1}),
+// and the sample weight s.  The path is replayed from its Philox key; after it ends
+// S_k = d_k + s_k L_{k+1} is formed backwards and dL/dalbedo += w dL/dI beta_k S_k.
+// dL/demission is added during the replay: w dL/dI beta_k A_k (cos_i/pi)(cos_l A/d^2).
+constexpr int kVtxWords = 8;
+
+__global__ __launch_bounds__(kRB) void k_backward(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
+                                                   const float *__restrict__ dl, double *__restrict__ g_albedo,
+                                                   double *__restrict__ g_emission, int levels, int maxv) {
+  extern __shared__ uint32_t lds[];
+  __shared__ int next_job;
+  uint32_t *stk = lds + threadIdx.x;
+  float *vtx = (float *)(lds + levels * kRB);                     // maxv * kVtxWords * kRB
+  float *gacc = vtx + (size_t)maxv * kVtxWords * kRB;              // nmat*3 albedo, then nlights*3 emission
+  const int ngrad = 3 * (P.nmat + P.nlights);
+  const int block = blockIdx.x;
+  const int spp = P.fsaa_x * P.fsaa_y;
+  const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
+  const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
+  const bool clip = P.cull_backfaces != 0;
+  const int lane = (int)__lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int t = threadIdx.x; t < ngrad; t += kRB) gacc[t] = 0.0f;
+  if (threadIdx.x == 0) next_job = 0;
+  __syncthreads();
+  const int njobs = kBlockPixels * spp;
+  auto V_ = [&](int k, int w) -> float & { return vtx[((size_t)k * kVtxWords + w) * kRB + threadIdx.x]; };
+
+  Path st;
+  int K = 0;                      // vertices recorded for the current path
+  float wdl[3] = {0, 0, 0};       // spw * dL/dI of the current pixel
+  bool exhausted = false;
+  auto finish = [&]() {
+    float Ln[3] = {0, 0, 0};
+    for (int k = K - 1; k >= 0; k--)
+      for (int c = 0; c < 3; c++) {
+        const float S = V_(k, 4 + c) + V_(k, 7) * Ln[c];
+        Ln[c] = V_(k, 1 + c) * S;
+        V_(k, 4 + c) = S;
+      }
+    float beta[3] = {1, 1, 1};
+    for (int k = 0; k < K; k++) {
+      const int m = __float_as_int(V_(k, 0));
+      const int kind = P.materials[m].kind;
+      for (int c = 0; c < 3; c++) {
+        if (kind != DR_METAL) atomicAdd(&gacc[3 * m + c], wdl[c] * beta[c] * V_(k, 4 + c));
+        beta[c] = beta[c] * (V_(k, 1 + c) * V_(k, 7));
+      }
+    }
+    st.job = -1;
+  };
+  auto advance = [&]() {
+    float A[3];
+    reflect_factor(P, st, A);
+    for (int c = 0; c < 3; c++) V_(K - 1, 1 + c) = A[c];
+    V_(K - 1, 7) = st.has_sample ? st.sweight : 0.0f;
+    if (!st.has_sample) { finish(); return; }
+    for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (A[c] * st.sweight);
+    st.o = st.p;
+    st.d = st.sdir;
+    st.restrict_tri = st.h;
+    st.layer++;
+    st.phase = 0;
+  };
+  for (;;) {
+    const bool want = (st.job < 0) && !exhausted;
+    const uint64_t wmask = __ballot(want);
+    if (wmask) {
+      const int leader = __ffsll((unsigned long long)wmask) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&next_job, __popcll(wmask));
+      base = __shfl(base, leader);
+      if (want) {
+        const int slot = base + __popcll(wmask & below);
+        if (slot < njobs) {
+          st.job = slot;
+          const int lp = slot / spp, s = slot - lp * spp;
+          K = 0;
+          if (!start_path(P, block, lp, s, samp0, st)) st.job = -1;
+          else {
+            const float *g = dl + 4 * ((size_t)kBlockPixels * block + lp);
+            for (int c = 0; c < 3; c++) wdl[c] = spw * g[c];
+          }
+        } else {
+          exhausted = true;
+        }
+      }
+    }
+    if (__ballot(st.job >= 0) == 0) {
+      if (__ballot(!exhausted) == 0) break;
+      continue;
+    }
+    if (st.job >= 0) {
+      Hit hh;
+      const bool hit = tree_cast(T, st.o, st.d, clip, st.restrict_tri, stk, kRB, hh);
+      if (st.phase == 0) {
+        if (!hit) finish();
+        else {
+          setup_vertex(P, shade, hh, st);
+          V_(K, 0) = __int_as_float(st.mid);
+          for (int c = 0; c < 4; c++) V_(K, 4 + c) = 0.0f;
+          K++;
+          if (!(st.kind == DR_LAMBERT && next_photon(P, st))) advance();
+        }
+      } else {
+        const V3 e = hh.p - st.p;
+        const bool lit = hit && (hh.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
+        if (lit) {
+          const dr_material M = P.materials[st.mid];
+          const float *E = P.lights[st.light].emission;
+          for (int c = 0; c < 3; c++) {
+            V_(K - 1, 4 + c) += st.gl_k * E[c];
+            atomicAdd(&gacc[3 * P.nmat + 3 * st.light + c], wdl[c] * st.beta[c] * M.albedo[c] * st.gl_k);
+          }
+        }
+        if (!next_photon(P, st)) advance();
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ngrad; t += kRB) {
+    const float v = gacc[t];
+    if (v != 0.0f) {
+      if (t < 3 * P.nmat) atomicAdd(&g_albedo[t], (double)v);
+      else atomicAdd(&g_emission[t - 3 * P.nmat], (double)v);
+    }
+  }
+}
+
+static int check_params(dr_scene s, const dr_render_params *p) {
+  if (!s || !p) return fail(DR_ERR_ARG, "render: null scene or parameters");
+  if (!s->has_materials) return fail(DR_ERR_ARG, "render: dr_scene_set_materials has not been called");
+  if (p->width < 1 || p->height < 1 || p->fsaa_x < 1 || p->fsaa_y < 1 || p->max_bounces < 0 || p->max_bounces > 64)
+    return fail(DR_ERR_ARG, "render: bad image size, subpixel grid or max_bounces");
+  if (p->nlights < 0 || p->nlights > kMaxLights || (p->nlights > 0 && !p->lights))
+    return fail(DR_ERR_ARG, "render: 0 <= nlights <= 16 with a light table required");
+  if (!p->materials || p->nmat <= s->max_mat) return fail(DR_ERR_ARG, "render: material table smaller than the ids in use");
+  if (!(p->rr_survive > 0.0f && p->rr_survive <= 1.0f)) return fail(DR_ERR_ARG, "render: rr_survive must be in (0, 1]");
+  if (s->max_depth > kMaxTreeDepth) return fail(DR_ERR_ARG, "render: octree deeper than the traversal stack");
+  return DR_OK;
+}
+
+static int block_count(const dr_render_params *p) { return ((p->width + 7) / 8) * ((p->height + 15) / 16); }
+
+}  // namespace dr
+
+extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
+                                void *stream) {
+  int rc = dr::check_params(s, p);
+  if (rc != DR_OK) return rc;
+  if (!frame || start_block < 0 || end_block > dr::block_count(p) || start_block > end_block)
+    return dr::fail(DR_ERR_ARG, "dr_render_blocks: bad frame or block range");
+  if (start_block == end_block) return DR_OK;
+  const int levels = s->max_depth > 0 ? s->max_depth : 1;
+  const size_t lds = ((size_t)levels * dr::kRB + (size_t)dr::kBlockPixels * dr::kRoundSamples * 3) * 4;
+  hipLaunchKernelGGL(dr::k_render, dim3(end_block - start_block), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree,
+                     (const float4 *)s->shade.p, *p, frame, start_block, levels);
+  DR_HIP_CHECK(hipGetLastError());
+  return DR_OK;
+}
+
+extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo,
+                           double *g_emission, void *stream) {
+  int rc = dr::check_params(s, p);
+  if (rc != DR_OK) return rc;
+  if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return dr::fail(DR_ERR_ARG, "dr_backward: null buffers");
+  const int levels = s->max_depth > 0 ? s->max_depth : 1;
+  const int maxv = p->max_bounces + 1;
+  const size_t lds = ((size_t)levels * dr::kRB + (size_t)maxv * dr::kVtxWords * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
+  if (lds > 160 * 1024) return dr::fail(DR_ERR_ARG, "dr_backward: path too long or too many materials for LDS");
+  if (lds > 64 * 1024) DR_HIP_CHECK(hipFuncSetAttribute((const void *)dr::k_backward,
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(dr::k_backward, dim3(dr::block_count(p)), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree,
+                     (const float4 *)s->shade.p, *p, dl_dframe, g_albedo, g_emission, levels, maxv);
+  DR_HIP_CHECK(hipGetLastError());
+  return DR_OK;
+}

@@ -1,0 +1,174 @@
+// Sampling and shading of the path estimator (device side).
+//
+// The renderer follows DumbRenderer::PixelRenderProcess (DumbRenderer.cu:174-443):
+// camera samples per subpixel, one Octree::cast per layer with the previous
+// layer's triangle excluded, light photons cast from the light toward the hit
+// point and counted when they first hit the same triangle or land within
+// sqrt(8 eps) of it, and the layer colour reflected up the path.  The
+// materials and the light are the plugins the benchmark scenes need (Lambert,
+// perfect mirror, Schlick-Fresnel metal, one-sided area quad light, Russian
+// roulette); random numbers are counter-based Philox4x32-10 keyed by
+// (seed) with counter (pixel, sample, layer << 8 | purpose, request), so any
+// schedule draws the same numbers for the same sample.  Every fp32 expression
+// here is evaluated in a fixed order without contraction, so rays, hits and
+// sample directions are bit-identical to a host evaluation of the same
+// expressions.
+#pragma once
+#include "dr_geom.h"
+#include "../../include/dumbray.h"
+
+namespace dr {
+
+constexpr float kInvPi = 0.31830988618379067154f;
+constexpr int kMaxLights = 16;
+
+DR_HD void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                         uint32_t out[4]) {
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Four uniforms in [0,1) (24-bit) for one (pixel, sample, layer, purpose).
+// purpose 0 = lens jitter, 1 = BSDF direction (u0, u1) + Russian roulette (u2), 16 + l = light l.
+DR_HD void rng4(const dr_render_params &P, uint32_t pix, uint32_t samp, uint32_t layer, uint32_t purpose, float u[4]) {
+  uint32_t o[4];
+  philox4x32_10(pix, samp, (layer << 8) | purpose, P.request, P.seed_lo, P.seed_hi, o);
+  for (int k = 0; k < 4; k++) u[k] = (float)(o[k] >> 8) * (1.0f / 16777216.0f);
+}
+
+// sin and cos of 2*pi*u, u in [0,1): quadrant from 4u, Taylor series on [0, pi/2).
+DR_HD void sincos_2pi(float u, float &s, float &c) {
+  const float x = u * 4.0f;
+  const int q = (int)x;
+  const float f = x - (float)q;
+  const float a = f * 1.57079637f;
+  const float a2 = a * a;
+  const float sa = a * (1.0f + a2 * (-0.166666667f + a2 * (0.00833333377f + a2 * (-0.000198412701f +
+                   a2 * (2.75573188e-06f + a2 * (-2.50521079e-08f))))));
+  const float ca = 1.0f + a2 * (-0.5f + a2 * (0.0416666679f + a2 * (-0.00138888892f + a2 * (2.48015876e-05f +
+                   a2 * (-2.75573188e-07f + a2 * 2.08767570e-09f)))));
+  switch (q & 3) {
+    case 0: s = sa; c = ca; break;
+    case 1: s = ca; c = -sa; break;
+    case 2: s = -sa; c = -ca; break;
+    default: s = -ca; c = sa; break;
+  }
+}
+
+DR_HD V3 normalized(V3 a) {
+  const float il = 1.0f / __builtin_sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);   // Vector3::normalized
+  return a * il;
+}
+
+// Camera sample of subpixel (i, j) of pixel (px, py): DumbRenderer.cu:190-232 (HEIGHT
+// boxing), SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:12-20 with
+// DumbRand::range = u*(max-min)+min), Camera transform (Transform.impl.h:166-168).
+DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, float u0, float u1, V3 &o, V3 &d) {
+  const float W = (float)P.width, H = (float)P.height;
+  const float ps = (1.0f / H);
+  const float pdx = ps / ((float)P.fsaa_x), pdy = ps / ((float)P.fsaa_y);
+  const float ppx = ((float)px - (W / 2.0f)) * ps;
+  const float ppy = ((H / 2.0f) - (float)py) * ps;
+  const float sx = ppx + (((float)i) + 0.5f) * pdx;
+  const float sy = ppy + (((float)j) + 0.5f) * pdy;
+  const float offx = pdx * 0.5f, offy = pdy * 0.5f;
+  const float lx = sx + ((u0 * (offx - (-offx))) + (-offx));
+  const float ly = sy + ((u1 * (offy - (-offy))) + (-offy));
+  const float lz = P.cam[12];
+  const float *C = P.cam;
+  o = v3(C[0], C[1], C[2]);
+  d = v3((lx * C[3] + ly * C[6]) + lz * C[9], (lx * C[4] + ly * C[7]) + lz * C[10], (lx * C[5] + ly * C[8]) + lz * C[11]);
+}
+
+// Unit geometric normal n = (b-a)x(c-a) normalised, turned to face the incoming ray.
+DR_HD V3 facing_normal(V3 n, V3 du) {
+  V3 nu = normalized(n);
+  if (dot(nu, du) > 0) nu = v3(-nu.x, -nu.y, -nu.z);
+  return nu;
+}
+
+// Cosine-weighted direction about unit n (orthonormal basis of Duff et al., JCGT 2017).
+DR_HD V3 cosine_dir(V3 n, float u1, float u2) {
+  const float sign = __builtin_copysignf(1.0f, n.z);
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  const V3 b1 = v3(1.0f + ((sign * n.x) * n.x) * a, sign * b, -sign * n.x);
+  const V3 b2 = v3(b, sign + (n.y * n.y) * a, -n.y);
+  const float r = __builtin_sqrtf(u1);
+  float s, c;
+  sincos_2pi(u2, s, c);
+  const float x = r * c, y = r * s, z = __builtin_sqrtf(1.0f - u1);
+  return ((b1 * x) + (b2 * y)) + (n * z);
+}
+
+// Vector3::reflectionOnUnitVector (Vector3.impl.h:393-396)
+DR_HD V3 reflect_unit(V3 v, V3 n) {
+  const float mul = (2.0f * dot(v, n));
+  return v3(v.x - (n.x * mul), v.y - (n.y * mul), v.z - (n.z * mul));
+}
+
+// Sampling decision at `layer` (requestIndirectSamples + Russian roulette).
+DR_HD bool next_sample(const dr_render_params &P, int kind, uint32_t pix, uint32_t samp, int layer, V3 nu, V3 du,
+                       V3 &dir, float &weight) {
+  if (layer >= P.max_bounces) return false;
+  float u[4];
+  rng4(P, pix, samp, (uint32_t)layer, 1, u);
+  weight = 1.0f;
+  if (layer >= P.rr_start) {
+    if (u[2] >= P.rr_survive) return false;
+    weight = 1.0f / P.rr_survive;
+  }
+  dir = (kind == DR_LAMBERT) ? cosine_dir(nu, u[0], u[1]) : reflect_unit(du, nu);
+  return true;
+}
+
+// Schlick factor of the metal for the reflected unit direction (DumbTools::fresnel, DumbStructs.cuh:136-140).
+DR_HD float schlick5(V3 nu, V3 dir) {
+  const float val = (1.0f - dot(nu, dir));
+  const float sq = (val * val);
+  return (sq * sq) * val;
+}
+
+// Area-quad light photon toward p: shadow ray (q, w), cosine at p and cos_l * A / d^2.
+DR_HD bool light_photon(const dr_render_params &P, const dr_light &L, int l, uint32_t pix, uint32_t samp, int layer,
+                        V3 p, V3 nu, V3 &q, V3 &w, float &cos_i, float &gl) {
+  float u[4];
+  rng4(P, pix, samp, (uint32_t)layer, 16u + (uint32_t)l, u);
+  const V3 cor = v3(L.corner[0], L.corner[1], L.corner[2]);
+  const V3 eu = v3(L.edge_u[0], L.edge_u[1], L.edge_u[2]), ev = v3(L.edge_v[0], L.edge_v[1], L.edge_v[2]);
+  q = (cor + eu * u[0]) + ev * u[1];
+  const V3 delta = p - q;
+  const float d2 = dot(delta, delta);
+  if (!(d2 > 0.0f)) return false;
+  w = delta * (1.0f / __builtin_sqrtf(d2));
+  const V3 nl = cross(eu, ev);
+  const float area = __builtin_sqrtf(dot(nl, nl));
+  const V3 nlu = nl * (1.0f / area);
+  const float cos_l = dot(nlu, w);
+  if (!(cos_l > 0.0f)) return false;
+  cos_i = -dot(nu, w);
+  if (!(cos_i > 0.0f)) return false;
+  gl = (cos_l * area) / d2;
+  return true;
+}
+
+// BlockBasedFrameBuffer<8,16>::blockPixelLocation (BlockBasedFrameBuffer.impl.cuh:125-143)
+DR_HD bool block_pixel(int width, int height, int block, int pixel, int &x, int &y) {
+  const int nbw = (width + 8 - 1) / 8;
+  const int by = block / nbw, bx = block - by * nbw;
+  const int oy = pixel / 8, ox = pixel - oy * 8;
+  x = 8 * bx + ox;
+  y = 16 * by + oy;
+  return x < width && y < height;
+}
+
+}  // namespace dr

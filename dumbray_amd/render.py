@@ -1,0 +1,173 @@
+"""Host mirror of the reference's render operator over the C ABI.
+
+``BlockBasedFrameBuffer`` keeps the reference's ``BlockBasedFrameBuffer<8,16>``
+layout (BlockBasedFrameBuffer.cuh:34-91, .impl.cuh:108-156): RGBA fp32, block-
+major, 128 pixels per block, in device memory.  ``DumbRenderer`` keeps the
+settings of ``DumbRenderer`` (DumbRenderer.cuh:20-31: max bounces, samples per
+pixel X/Y, ignore backfaces; HEIGHT boxing) and its ``renderBlocksGPU``
+(DumbRenderer.cu:87-113), plus ``backward`` (the adjoint, dr_backward).
+Materials and lights live in device tables owned by the renderer, so
+``albedo`` and ``emission`` are torch views that an optimiser can update in
+place between renders.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import DumbRayError, check, lib
+from .octree import DeviceScene
+
+DR_LAMBERT, DR_MIRROR, DR_METAL = 0, 1, 2
+KINDS = {"lambert": DR_LAMBERT, "mirror": DR_MIRROR, "metal": DR_METAL}
+BLOCK_W, BLOCK_H = 8, 16
+BLOCK_SIZE = BLOCK_W * BLOCK_H
+DUMB_RENDERER_BOUNCE_LIMIT = 64      # DumbRenderer.cuh:7
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("fsaa_x", ctypes.c_int32),
+                ("fsaa_y", ctypes.c_int32), ("max_bounces", ctypes.c_int32), ("rr_start", ctypes.c_int32),
+                ("rr_survive", ctypes.c_float), ("cull_backfaces", ctypes.c_int32), ("seed_lo", ctypes.c_uint32),
+                ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
+                ("cam", ctypes.c_float * 13), ("materials", ctypes.c_void_p), ("lights", ctypes.c_void_p),
+                ("nmat", ctypes.c_int32), ("nlights", ctypes.c_int32)]
+
+
+def _vp(x):
+    return ctypes.c_void_p(x)
+
+
+def block_count(width: int, height: int) -> int:
+    return ((width + BLOCK_W - 1) // BLOCK_W) * ((height + BLOCK_H - 1) // BLOCK_H)
+
+
+def block_to_image_index(width: int, height: int) -> np.ndarray:
+    """For every row-major pixel (y, x), its row in the block-major buffer
+    (BlockBasedFrameBuffer::pixelBlockLocation, BlockBasedFrameBuffer.impl.cuh:108-123)."""
+    nbw = (width + BLOCK_W - 1) // BLOCK_W
+    y, x = np.mgrid[0:height, 0:width]
+    block = (y // BLOCK_H) * nbw + (x // BLOCK_W)
+    pixel = (y % BLOCK_H) * BLOCK_W + (x % BLOCK_W)
+    return (block * BLOCK_SIZE + pixel).reshape(-1)
+
+
+class BlockBasedFrameBuffer:
+    """Device BlockBasedFrameBuffer<8,16>: data is a float32 tensor (blocks*128, 4)."""
+
+    def __init__(self, width: int, height: int, device="cuda"):
+        import torch
+
+        self.width, self.height = int(width), int(height)
+        self.data = torch.zeros((block_count(self.width, self.height) * BLOCK_SIZE, 4), dtype=torch.float32,
+                                device=device)
+        self._idx = torch.from_numpy(block_to_image_index(self.width, self.height)).to(self.data.device)
+
+    def getBlockSize(self) -> int:
+        return BLOCK_SIZE
+
+    def getBlockCount(self) -> int:
+        return block_count(self.width, self.height)
+
+    def getSize(self):
+        return self.width, self.height
+
+    def to_image(self):
+        """(H, W, 4) row-major view of the frame."""
+        return self.data[self._idx].view(self.height, self.width, 4)
+
+    def from_image(self, img):
+        """Block-major (blocks*128, 4) tensor from an (H, W, C) image (C = 3 or 4)."""
+        import torch
+
+        img = img.reshape(self.height * self.width, -1)
+        out = torch.zeros_like(self.data)
+        out[self._idx, : img.shape[1]] = img.to(out.dtype)
+        return out
+
+
+class DumbRenderer:
+    def __init__(self, scene: DeviceScene, mat_id, materials, lights, camera, width: int, height: int,
+                 samples_per_pixel=(1, 1), max_bounces: int = 2, rr_start: int = 1 << 30, rr_survive: float = 1.0,
+                 ignore_backfaces: bool = True, seed: int = 0x5EED, device="cuda"):
+        import torch
+
+        self.scene = scene
+        self.width, self.height = int(width), int(height)
+        self.fsaa_x, self.fsaa_y = (max(1, int(samples_per_pixel[0])), max(1, int(samples_per_pixel[1])))
+        self.max_bounces = min(int(max_bounces), DUMB_RENDERER_BOUNCE_LIMIT)
+        self.rr_start, self.rr_survive = int(min(rr_start, 1 << 30)), float(rr_survive)
+        self.ignore_backfaces = bool(ignore_backfaces)
+        self.seed = int(seed)
+        self.camera = camera
+        mid = np.ascontiguousarray(np.asarray(mat_id, dtype=np.int32))
+        check(lib().dr_scene_set_materials(scene.handle, mid.ctypes.data_as(ctypes.c_void_p), int(mid.shape[0])),
+              "dr_scene_set_materials")
+        mt = np.zeros((len(materials), 4), np.float32)
+        for i, m in enumerate(materials):
+            mt[i, 0] = np.array([KINDS[m["kind"]]], np.int32).view(np.float32)[0]
+            mt[i, 1:4] = m["albedo"]
+        lt = np.zeros((max(len(lights), 1), 12), np.float32)
+        for i, l in enumerate(lights):
+            lt[i] = list(l["corner"]) + list(l["edge_u"]) + list(l["edge_v"]) + list(l["emission"])
+        self.material_table = torch.from_numpy(mt).to(device)
+        self.light_table = torch.from_numpy(lt).to(device)
+        self.nmat, self.nlights = len(materials), len(lights)
+
+    # parameters an optimiser may update in place
+    @property
+    def albedo(self):
+        return self.material_table[:, 1:4]
+
+    @property
+    def emission(self):
+        return self.light_table[: self.nlights, 9:12]
+
+    def params(self, iteration: int = 1, request: int = 0) -> RenderParams:
+        p = RenderParams()
+        p.width, p.height, p.fsaa_x, p.fsaa_y = self.width, self.height, self.fsaa_x, self.fsaa_y
+        p.max_bounces, p.rr_start, p.rr_survive = self.max_bounces, self.rr_start, self.rr_survive
+        p.cull_backfaces = int(self.ignore_backfaces)
+        p.seed_lo, p.seed_hi = self.seed & 0xFFFFFFFF, (self.seed >> 32) & 0xFFFFFFFF
+        p.request, p.iteration = int(request), int(iteration)
+        cam = self.camera.as_array()
+        for k in range(13):
+            p.cam[k] = float(cam[k])
+        p.materials = self.material_table.data_ptr()
+        p.lights = self.light_table.data_ptr()
+        p.nmat, p.nlights = self.nmat, self.nlights
+        return p
+
+    def renderBlocksGPU(self, frame: BlockBasedFrameBuffer, start_block: int, end_block: int, iteration: int = 1,
+                        stream=None):
+        import torch
+
+        if frame.getSize() != (self.width, self.height):
+            raise DumbRayError("renderBlocksGPU: frame size differs from the renderer's")
+        p = self.params(iteration)
+        s = stream if stream is not None else torch.cuda.current_stream(frame.data.device)
+        check(lib().dr_render_blocks(self.scene.handle, ctypes.byref(p), _vp(frame.data.data_ptr()), int(start_block),
+                                     int(end_block), _vp(s.cuda_stream)), "dr_render_blocks")
+
+    def render(self, frame: BlockBasedFrameBuffer | None = None, iteration: int = 1, stream=None):
+        if frame is None:
+            frame = BlockBasedFrameBuffer(self.width, self.height, self.material_table.device)
+        self.renderBlocksGPU(frame, 0, frame.getBlockCount(), iteration, stream)
+        return frame
+
+    def backward(self, dl_dframe, iteration: int = 1, stream=None):
+        """Renderer::backward: (dL/dalbedo (nmat,3), dL/demission (nlights,3)) as float64 tensors for
+        the loss gradient dl_dframe (block-major (blocks*128, 4), alpha ignored)."""
+        import torch
+
+        dl = dl_dframe.contiguous().to(torch.float32)
+        if dl.shape != (block_count(self.width, self.height) * BLOCK_SIZE, 4):
+            raise DumbRayError("backward: dl_dframe must have the frame's block layout")
+        ga = torch.zeros((self.nmat, 3), dtype=torch.float64, device=dl.device)
+        ge = torch.zeros((max(self.nlights, 1), 3), dtype=torch.float64, device=dl.device)
+        p = self.params(iteration)
+        s = stream if stream is not None else torch.cuda.current_stream(dl.device)
+        check(lib().dr_backward(self.scene.handle, ctypes.byref(p), _vp(dl.data_ptr()), _vp(ga.data_ptr()),
+                                _vp(ge.data_ptr()), _vp(s.cuda_stream)), "dr_backward")
+        return ga, ge[: self.nlights]

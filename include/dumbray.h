@@ -76,6 +76,64 @@ int64_t dr_scene_device_bytes(dr_scene s);
 int dr_cast(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude, int32_t *out_id,
             float *out_t, float *out_point, void *stream);
 
+/* ---------------------------------------------------------------- path tracing */
+
+/* Material kinds (plugins of Material<BakedTriFace>, Material.cuh:115-123). */
+enum { DR_LAMBERT = 0, DR_MIRROR = 1, DR_METAL = 2 };
+
+/* One material: Lambert albedo / mirror reflectance / Schlick metal F0 (16 B). */
+typedef struct {
+  int32_t kind;
+  float albedo[3];
+} dr_material;
+
+/* One-sided area quad light emitting on the (edge_u x edge_v) side (48 B);
+ * the Light plugin interface is Light::getVertexPhotons (Light.cuh:53-59). */
+typedef struct {
+  float corner[3], edge_u[3], edge_v[3], emission[3];
+} dr_light;
+
+/* Per-call render parameters (DumbRenderer::PixelRenderProcess::SceneConfiguration,
+ * DumbRenderer.cuh:80-101, plus the estimator's seeds).  Boxing is HEIGHT based. */
+typedef struct {
+  int32_t width, height;          /* image size in pixels */
+  int32_t fsaa_x, fsaa_y;         /* subpixel grid = samples per pixel per iteration */
+  int32_t max_bounces;            /* DumbRenderer maxBounces (<= 64, DumbRenderer.cuh:7) */
+  int32_t rr_start;               /* Russian roulette at layers >= rr_start */
+  float rr_survive;               /* survival probability (1 disables) */
+  int32_t cull_backfaces;         /* DumbRenderer ignoreBackfaces */
+  uint32_t seed_lo, seed_hi;      /* Philox key */
+  uint32_t request;               /* Philox counter word 3 (batched requests) */
+  int32_t iteration;              /* 1-based; 1 sets the frame, k > 1 blends with 1/k (DumbRenderer.cu:64) */
+  float cam[13];                  /* camera: position, right, up, forward, focal = 0.5/tan(fov/2) */
+  const dr_material *materials;   /* device table [nmat] */
+  const dr_light *lights;         /* device table [nlights] */
+  int32_t nmat, nlights;          /* nlights <= 16 */
+} dr_render_params;
+
+/* Attach per-triangle material ids (host int32[ntris], ntris = the octree's triangle
+ * count; the scene keeps a device copy).  Needed before rendering. */
+int dr_scene_set_materials(dr_scene s, const int32_t *mat_id, int ntris);
+
+/* DumbRenderer::renderBlocksGPU (DumbRenderer.cu:87-113, override of
+ * BlockRenderer::renderBlocksGPU, BlockRenderer.cuh:43-44): renders framebuffer
+ * blocks [start_block, end_block) of one iteration into `frame`, a device
+ * BlockBasedFrameBuffer<8,16> (RGBA fp32, pixel (x, y) of block b at
+ * frame[4 * (128 * b + 8 * (y % 16) + x % 8)], blocks numbered row by row,
+ * BlockBasedFrameBuffer.impl.cuh:108-156).  Iteration 1 sets pixels, iteration k
+ * blends colour * (1/k) + old * (1 - 1/k).  Asynchronous on `stream`. */
+int dr_render_blocks(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
+                     void *stream);
+
+/* Renderer::backward: adjoint of one dr_render_blocks iteration over the whole image
+ * by path replay (each path is re-traced from its Philox key, nothing is stored
+ * between passes).  dl_dframe: device, same block layout as `frame` (alpha ignored).
+ * Adds dL/dalbedo (Lambert albedo and mirror reflectance; device double[nmat*3]) and
+ * dL/demission (device double[nlights*3]); the caller zeroes them.  Per-block
+ * reductions in LDS precede one FP64 atomic per block and parameter. */
+int dr_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo, double *g_emission,
+                void *stream);
+
 /* dr_cast plus per-ray work counters of the traversal that ran (measurement only;
  * no counterpart in the reference, which prints timings only): out_counts is a
  * device int32[n*4] receiving, per ray, the groups expanded (8 child box tests,

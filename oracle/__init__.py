@@ -68,12 +68,112 @@ def lib():
     return _LIB
 
 
+class OraMaterial(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("albedo", ctypes.c_float * 3)]
+
+
+class OraLight(ctypes.Structure):
+    _fields_ = [("corner", ctypes.c_float * 3), ("edge_u", ctypes.c_float * 3), ("edge_v", ctypes.c_float * 3),
+                ("emission", ctypes.c_float * 3)]
+
+
+class OraRenderCfg(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("fsaa_x", ctypes.c_int32),
+                ("fsaa_y", ctypes.c_int32), ("max_bounces", ctypes.c_int32), ("rr_start", ctypes.c_int32),
+                ("rr_survive", ctypes.c_float), ("cull_backfaces", ctypes.c_int32), ("seed_lo", ctypes.c_uint32),
+                ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
+                ("cam", ctypes.c_float * 13)]
+
+
+KINDS = {"lambert": 0, "mirror": 1, "metal": 2}
+
+
 def _bind_render(L):
     vp = ctypes.c_void_p
     L.ora_render.restype = ctypes.c_int
-    L.ora_render.argtypes = [vp, vp, vp, vp, ctypes.c_int]
+    L.ora_render.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, ctypes.c_int]
     L.ora_backward.restype = ctypes.c_int
-    L.ora_backward.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int]
+    L.ora_backward.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
+    L.ora_philox.argtypes = [vp, vp, vp]
+    L.ora_sincos_2pi.argtypes = [ctypes.c_float, vp, vp]
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().ora_philox(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
+
+
+def sincos_2pi(u):
+    s = ctypes.c_float()
+    c = ctypes.c_float()
+    lib().ora_sincos_2pi(ctypes.c_float(u), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+class Renderer:
+    """Oracle path estimator over an oracle Octree (same settings as dumbray_amd.DumbRenderer)."""
+
+    def __init__(self, tree, mat_id, materials, lights, cam_array, width, height, samples_per_pixel=(1, 1),
+                 max_bounces=2, rr_start=1 << 30, rr_survive=1.0, ignore_backfaces=True, seed=0x5EED):
+        self.tree = tree
+        self.mat_id = np.ascontiguousarray(mat_id, np.int32)
+        self.mats = (OraMaterial * max(len(materials), 1))()
+        for i, m in enumerate(materials):
+            self.mats[i].kind = KINDS[m["kind"]]
+            for c in range(3):
+                self.mats[i].albedo[c] = float(np.float32(m["albedo"][c]))
+        self.nmat = len(materials)
+        self.lights = (OraLight * max(len(lights), 1))()
+        for i, l in enumerate(lights):
+            for c in range(3):
+                self.lights[i].corner[c] = l["corner"][c]
+                self.lights[i].edge_u[c] = l["edge_u"][c]
+                self.lights[i].edge_v[c] = l["edge_v"][c]
+                self.lights[i].emission[c] = l["emission"][c]
+        self.nlights = len(lights)
+        cfg = OraRenderCfg()
+        cfg.width, cfg.height = int(width), int(height)
+        cfg.fsaa_x, cfg.fsaa_y = int(samples_per_pixel[0]), int(samples_per_pixel[1])
+        cfg.max_bounces, cfg.rr_start, cfg.rr_survive = int(max_bounces), int(min(rr_start, 1 << 30)), float(rr_survive)
+        cfg.cull_backfaces = int(ignore_backfaces)
+        cfg.seed_lo, cfg.seed_hi = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+        cfg.request, cfg.iteration = 0, 1
+        for k in range(13):
+            cfg.cam[k] = float(cam_array[k])
+        self.cfg = cfg
+
+    def set_albedo(self, i, rgb):
+        for c in range(3):
+            self.mats[i].albedo[c] = float(np.float32(rgb[c]))
+
+    def set_emission(self, i, rgb):
+        for c in range(3):
+            self.lights[i].emission[c] = float(np.float32(rgb[c]))
+
+    def render(self, iteration=1, fp64=False, nthreads=0, request=0):
+        self.cfg.iteration = int(iteration)
+        self.cfg.request = int(request)
+        H, W = self.cfg.height, self.cfg.width
+        out = np.zeros((H, W, 3), np.float64 if fp64 else np.float32)
+        rc = lib().ora_render(self.tree.h, self.mat_id.ctypes.data, ctypes.addressof(self.mats), self.nmat,
+                              ctypes.addressof(self.lights), self.nlights, ctypes.addressof(self.cfg),
+                              None if fp64 else out.ctypes.data, out.ctypes.data if fp64 else None, int(nthreads))
+        assert rc == 0
+        return out
+
+    def backward(self, dl_dimage, iteration=1, nthreads=0):
+        self.cfg.iteration = int(iteration)
+        dl = np.ascontiguousarray(dl_dimage, np.float32).reshape(self.cfg.height, self.cfg.width, 3)
+        ga = np.zeros((max(self.nmat, 1), 3), np.float64)
+        ge = np.zeros((max(self.nlights, 1), 3), np.float64)
+        rc = lib().ora_backward(self.tree.h, self.mat_id.ctypes.data, ctypes.addressof(self.mats), self.nmat,
+                                ctypes.addressof(self.lights), self.nlights, ctypes.addressof(self.cfg),
+                                dl.ctypes.data, ga.ctypes.data, ge.ctypes.data, int(nthreads))
+        assert rc == 0
+        return ga[: self.nmat], ge[: self.nlights]
 
 
 def _f32(a):

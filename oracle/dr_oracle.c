@@ -606,6 +606,16 @@ void ora_brute_cast(const float *tris, int ntris, const float *rays, int n, int 
 	run_jobs(&p, n, nthreads, brute_worker);
 }
 
+/* single-ray cast for the path estimator (dr_oracle_render.c) */
+int ora_cast1(const ora_octree *t, const float *o, const float *d, int clip, int32_t exclude, int32_t *id, float *dist,
+              float *point) {
+	int32_t nb = 0, nr = 0, nt = 0; v3 hp = V(0, 0, 0);
+	int r = oct_cast(t, ld3(o), ld3(d), clip, exclude, id, dist, &hp, &nb, &nr, &nt);
+	if (r) { point[0] = hp.x; point[1] = hp.y; point[2] = hp.z; }
+	return r;
+}
+const float *ora_octree_tri(const ora_octree *t, int32_t id) { return &t->tris[id].a.x; }
+
 /* ---- primitive probes ---- */
 int ora_tri_cast(const float *tri, const float *o, const float *d, int clip, float *t, float *p) {
 	tri_t tr = ldtri(tri); v3 h; float dd;

@@ -1,0 +1,68 @@
+/*
+ * CPU oracle of the path estimator (TEST INFRASTRUCTURE ONLY, see dr_oracle.h).
+ *
+ * Restates DumbRenderer::PixelRenderProcess (DumbRenderer.cu:174-443): per
+ * subpixel a camera sample, per layer one Octree::cast with the previous
+ * layer's triangle excluded (:285, :365-366), photons cast from each light
+ * toward the hit point and counted as lit when they first hit the same
+ * triangle or land within sqrt(8 eps) of it (:388-395), indirect samples
+ * requested from the material (:369-385), and the parent layer accumulating
+ * getReflectedColor(child photon) * sampleWeight (:330-345).  Layer colours
+ * are accumulated bottom-up, as the reference does.
+ *
+ * The materials and the light here are the plugins the benchmark scenes need
+ * (none exists in the reference): Lambert (albedo/pi, cosine-weighted
+ * sampling), perfect mirror, Schlick-Fresnel metal (DumbTools::fresnel,
+ * DumbStructs.cuh:136-140) and a one-sided area quad light, plus Russian
+ * roulette with a fixed survival probability.  Random numbers come from
+ * Philox4x32-10 (Salmon et al., SC'11) keyed by (seed) with counter
+ * (pixel, sample, layer<<8 | purpose, request), replacing the reference's
+ * per-thread XORWOW, so every sample is reproducible on any schedule.
+ */
+#ifndef DR_ORACLE_RENDER_H
+#define DR_ORACLE_RENDER_H
+#include <stdint.h>
+#include "dr_oracle.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_LAMBERT = 0, ORA_MIRROR = 1, ORA_METAL = 2 };
+
+typedef struct {
+	int32_t kind;
+	float albedo[3];    /* Lambert albedo, mirror reflectance, metal F0 */
+} ora_material;
+
+typedef struct {
+	float corner[3], edge_u[3], edge_v[3], emission[3];   /* emits on the (edge_u x edge_v) side */
+} ora_light;
+
+typedef struct {
+	int32_t width, height, fsaa_x, fsaa_y, max_bounces, rr_start;
+	float rr_survive;
+	int32_t cull_backfaces;
+	uint32_t seed_lo, seed_hi, request;
+	int32_t iteration;              /* 1-based; samples of iteration k are numbered from (k-1)*fsaa_x*fsaa_y */
+	float cam[13];                  /* position, right, up, forward, focal (0.5 / tan(fov/2)) */
+} ora_render_cfg;
+
+/* Pixel colours (row-major, H x W x 3) of one iteration.  fp64 != 0 accumulates
+ * all shading in double (geometry and sampling stay fp32, so paths are identical). */
+int ora_render(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat, const ora_light *lights,
+               int nlights, const ora_render_cfg *cfg, float *out_rgb, double *out_rgb64, int nthreads);
+/* FP64 adjoint of the same estimator: given dL/dimage (H x W x 3), accumulates
+ * dL/dalbedo (nmat x 3, Lambert albedo and mirror reflectance) and
+ * dL/demission (nlights x 3).  Paths are replayed from their Philox keys. */
+int ora_backward(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
+                 const ora_light *lights, int nlights, const ora_render_cfg *cfg, const float *dl_dimage,
+                 double *g_albedo, double *g_emission, int nthreads);
+
+/* Philox4x32-10 block and the sin/cos of 2*pi*u used by the samplers (probes for tests). */
+void ora_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void ora_sincos_2pi(float u, float *s, float *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
