@@ -58,6 +58,123 @@ def read_traffic(path):
         return None
 
 
+def _event_ms(fn, k, stream):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / k
+
+
+def _device_scene(D, tris):
+    t = D.Octree(16, 12)
+    t.push(tris)
+    t.build()
+    return t.upload()
+
+
+def bench_path_trace(D, S, stream, cpu_seconds, with_cpu):
+    """1920x1080, 16 spp (4x4), max depth 8, Russian roulette from depth 3 on three displaced
+    level-7 icospheres (983,040 tris) + ground quad, Lambert / mirror / Schlick metal, two area lights."""
+    import torch
+    sc = S.icosphere_scene(level=7, seed=3)
+    ds = _device_scene(D, sc.tris)
+    W, H, spp, depth = 1920, 1080, (4, 4), 8
+    g = D.DumbRenderer(ds, sc.mat_id, sc.materials, sc.lights, sc.camera, W, H, spp, depth, rr_start=3,
+                       rr_survive=0.75)
+    frame = D.BlockBasedFrameBuffer(W, H)
+    ms = _event_ms(lambda: g.render(frame), 3, stream)
+    out = {"workload": f"path trace {W}x{H}, 16 spp, depth 8, RR from 3 (survive 0.75), {len(sc.tris)} tris, "
+                       "Lambert/mirror/Schlick metal, 2 area lights",
+           "ms_per_frame": round(ms, 3), "mpix_spp_s": round(W * H * 16 / ms / 1e3, 2)}
+    log(f"path trace 1080p16: {ms:.2f} ms/frame, {out['mpix_spp_s']} Mpix*spp/s")
+    if with_cpu:
+        import oracle
+        ot = oracle.Octree(sc.tris, 16, 12)
+        threads = min(os.cpu_count() or 1, 16)
+        o = oracle.Renderer(ot, sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(), W, H, spp, depth,
+                            rr_start=3, rr_survive=0.75)
+        # a band of rows through the middle of the frame, grown until ~cpu_seconds/4
+        rows = 8
+        while True:
+            r0 = H // 2 - rows // 2
+            t0 = time.perf_counter()
+            o.render(nthreads=threads, rows=(r0, r0 + rows))
+            spent = time.perf_counter() - t0
+            if spent >= cpu_seconds / 4 or rows >= 256:
+                break
+            rows *= 2
+        out["cpu_oracle"] = {"value": round(W * rows * 16 / spent / 1e6, 4), "unit": "Mpix*spp/s", "cores": threads,
+                             "kind": "port", "sample": f"rows {r0}..{r0 + rows - 1} of the {W}x{H} frame at 16 spp "
+                                                       f"through the oracle estimator, {spent:.1f} s"}
+        log(f"path trace cpu oracle: {out['cpu_oracle']}")
+    return out
+
+
+def bench_small(D, S, stream):
+    """256x256, 4 spp (2x2), depth 4, Lambert only: ground quad + 8 UV spheres, one 1x1 area light."""
+    sc = S.spheres_scene(8, seed=0)
+    g = D.DumbRenderer(_device_scene(D, sc.tris), sc.mat_id, sc.materials, sc.lights, sc.camera, 256, 256, (2, 2), 4)
+    frame = D.BlockBasedFrameBuffer(256, 256)
+    ms = _event_ms(lambda: g.render(frame), 10, stream)
+    return {"workload": "path trace 256x256, 4 spp, depth 4, 2050 tris", "ms_per_frame": round(ms, 4),
+            "mpix_spp_s": round(256 * 256 * 4 / ms / 1e3, 2)}
+
+
+def bench_adjoint(D, S, stream, adam_steps):
+    """512x512, 16 spp, depth 4; 64 UV spheres (64 Lambert materials) + ground, 4 area lights.
+    Forward render vs dr_backward time, then Adam (lr 0.01) on albedo + emission from a
+    random init toward a target rendered from the hidden seeded parameters."""
+    import torch
+    from dumbray_amd.autograd import render
+    sc = S.spheres_scene(64, seed=5, n_lights=4, ground_material=False)
+    W, H = 512, 512
+    g = D.DumbRenderer(_device_scene(D, sc.tris), sc.mat_id, sc.materials, sc.lights, sc.camera, W, H, (4, 4), 4)
+    frame = D.BlockBasedFrameBuffer(W, H)
+    fwd = _event_ms(lambda: g.render(frame), 5, stream)
+    dl = torch.randn_like(frame.data)
+    bwd = _event_ms(lambda: g.backward(dl), 5, stream)
+    out = {"workload": f"adjoint {W}x{H}, 16 spp, depth 4, {len(sc.tris)} tris, 64 materials, 4 lights",
+           "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3), "ratio": round(bwd / fwd, 3)}
+    log(f"adjoint: forward {fwd:.2f} ms, backward {bwd:.2f} ms, ratio {bwd / fwd:.3f}")
+    if adam_steps > 0:
+        hidden_a = g.albedo.clone()
+        hidden_e = g.emission.clone()
+        target = render(hidden_a, hidden_e, g).detach()
+        rng = torch.Generator(device="cpu").manual_seed(1234)
+        alb = torch.rand(hidden_a.shape, generator=rng).mul(0.6).add(0.2).cuda().requires_grad_(True)
+        em = torch.rand(hidden_e.shape, generator=rng).mul(10.0).add(5.0).cuda().requires_grad_(True)
+        opt = torch.optim.Adam([alb, em], lr=0.01)
+        losses = []
+        t0 = time.perf_counter()
+        for step in range(adam_steps):
+            g.seed = 0x5EED + 1 + step                 # fresh Philox key per step
+            img = render(alb, em, g)
+            loss = ((img[:, :3] - target[:, :3]) ** 2).sum()
+            opt.zero_grad()
+            loss.backward()
+            opt.step()
+            with torch.no_grad():
+                alb.clamp_(0.0, 1.0)
+                em.clamp_(min=0.0)
+            losses.append(float(loss))
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        g.seed = 0x5EED
+        rmse = lambda a, b: float(torch.sqrt(((a - b) ** 2).mean()))
+        out["inverse_rendering"] = {"adam_steps": adam_steps, "lr": 0.01, "seconds": round(secs, 2),
+                                    "loss_first": round(losses[0], 6), "loss_last": round(losses[-1], 6),
+                                    "albedo_rmse_last": round(rmse(alb.detach(), hidden_a), 5),
+                                    "emission_rmse_last": round(rmse(em.detach(), hidden_e), 5)}
+        log(f"inverse rendering: {out['inverse_rendering']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +187,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (pmc calibration)")
     ap.add_argument("--profile", action="store_true",
                     help="only the timed incoherent casts (+ calibration copy): for rocprofv3 runs")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracing and adjoint legs")
+    ap.add_argument("--adam-steps", type=int, default=200)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -190,6 +309,12 @@ def main():
     valu = {"achieved": round(ops / (kB_ms / 1e3) / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "T lane-ops/s", "frac": round(ops / (kB_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)}
 
+    render = None
+    if not args.no_render:
+        render = {"path_trace_1080p": bench_path_trace(D, S, stream, args.cpu_seconds, not args.no_cpu),
+                  "path_trace_256": bench_small(D, S, stream),
+                  "adjoint_512": bench_adjoint(D, S, stream, args.adam_steps)}
+
     line = {
         "metric": "closest-hit Mrays/s & Mpix·spp/s at 1080p; adjoint/forward time ratio",
         "value": round(value, 2),
@@ -215,6 +340,9 @@ def main():
         "reference_work_per_ray": per_ray,
         "oracle_sample_mismatches": parity,
         "cpu_baseline": cpu,
+        "mpix_spp_s_1080p": render["path_trace_1080p"]["mpix_spp_s"] if render else None,
+        "adjoint_forward_ratio": render["adjoint_512"]["ratio"] if render else None,
+        "render": render,
     }
     if cpu:
         line["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -308,22 +308,22 @@ class Scene:
     camera: Camera | None = None
 
 
-def spheres_scene(n_spheres: int = 8, seed: int = 0, n_lights: int = 1, per_sphere_material: bool = True) -> Scene:
+def spheres_scene(n_spheres: int = 8, seed: int = 0, n_lights: int = 1, ground_material: bool = True) -> Scene:
     """Ground quad (2 tris) + n UV spheres (16x8, 256 tris each), centres U[-4,4]^3,
     radii U(0.3,1.0), Lambert albedo U(0.2,0.8)^3 per sphere; area-light quads
-    1x1 at y=5 with emission 10."""
+    1x1 at y=5 with emission 10.  ground_material=False gives the ground sphere 0's
+    material, so there is exactly one material per sphere."""
     rng = np.random.Generator(np.random.Philox(seed))
     parts = [ground_quad(-4.0 - 1.0, 12.0)]
     mids = [np.zeros(2, np.int32)]
-    mats = [dict(kind="lambert", albedo=(0.5, 0.5, 0.5))]
+    mats = [dict(kind="lambert", albedo=(0.5, 0.5, 0.5))] if ground_material else []
     for s in range(n_spheres):
         c = rng.uniform(-4.0, 4.0, 3)
         r = rng.uniform(0.3, 1.0)
         alb = rng.uniform(0.2, 0.8, 3).astype(np.float32)
         parts.append(uv_sphere(c, r))
-        mid = len(mats) if per_sphere_material else 1
-        if per_sphere_material or len(mats) == 1:
-            mats.append(dict(kind="lambert", albedo=tuple(float(x) for x in alb)))
+        mid = len(mats)
+        mats.append(dict(kind="lambert", albedo=tuple(float(x) for x in alb)))
         mids.append(np.full(256, mid, np.int32))
     lights = []
     for k in range(n_lights):

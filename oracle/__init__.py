@@ -91,7 +91,8 @@ KINDS = {"lambert": 0, "mirror": 1, "metal": 2}
 def _bind_render(L):
     vp = ctypes.c_void_p
     L.ora_render.restype = ctypes.c_int
-    L.ora_render.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, ctypes.c_int]
+    L.ora_render.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int]
     L.ora_backward.restype = ctypes.c_int
     L.ora_backward.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
     L.ora_philox.argtypes = [vp, vp, vp]
@@ -153,14 +154,17 @@ class Renderer:
         for c in range(3):
             self.lights[i].emission[c] = float(np.float32(rgb[c]))
 
-    def render(self, iteration=1, fp64=False, nthreads=0, request=0):
+    def render(self, iteration=1, fp64=False, nthreads=0, request=0, rows=None):
+        """Image (rows x W x 3); rows = (row0, row1) renders only that band of the frame."""
         self.cfg.iteration = int(iteration)
         self.cfg.request = int(request)
         H, W = self.cfg.height, self.cfg.width
-        out = np.zeros((H, W, 3), np.float64 if fp64 else np.float32)
+        r0, r1 = (0, H) if rows is None else (int(rows[0]), int(rows[1]))
+        out = np.zeros((r1 - r0, W, 3), np.float64 if fp64 else np.float32)
         rc = lib().ora_render(self.tree.h, self.mat_id.ctypes.data, ctypes.addressof(self.mats), self.nmat,
                               ctypes.addressof(self.lights), self.nlights, ctypes.addressof(self.cfg),
-                              None if fp64 else out.ctypes.data, out.ctypes.data if fp64 else None, int(nthreads))
+                              None if fp64 else out.ctypes.data, out.ctypes.data if fp64 else None, r0, r1,
+                              int(nthreads))
         assert rc == 0
         return out
 

@@ -197,7 +197,7 @@ static int photon_lit(const rctx *C, v3 q, v3 w, int32_t h, v3 p) {
 /* threading                                                           */
 /* ------------------------------------------------------------------ */
 typedef struct {
-	const rctx *C; int y0, y1; float *out; double *out64; const float *dl; double *ga, *ge; int mode;
+	const rctx *C; int y0, y1, ybase; float *out; double *out64; const float *dl; double *ga, *ge; int mode;
 } rjob;
 static void *render_worker(void *arg) {
 	rjob *J = (rjob *)arg;
@@ -205,15 +205,16 @@ static void *render_worker(void *arg) {
 	const ora_render_cfg *cfg = C->cfg;
 	for (int py = J->y0; py < J->y1; py++)
 		for (int px = 0; px < cfg->width; px++) {
-			const size_t k = (size_t)py * cfg->width + px;
+			const size_t k = (size_t)(py - J->ybase) * cfg->width + px;
 			if (J->mode == 0) pixel_color_f(C, px, py, J->out + 3 * k);
 			else if (J->mode == 1) pixel_color_d(C, px, py, J->out64 + 3 * k);
 			else pixel_backward_d(C, px, py, J->dl + 3 * k, J->ga, J->ge);
 		}
 	return NULL;
 }
-static int run(const rctx *C, int mode, float *out, double *out64, const float *dl, double *ga, double *ge, int nth) {
-	const int H = C->cfg->height;
+static int run(const rctx *C, int mode, float *out, double *out64, const float *dl, double *ga, double *ge, int nth,
+               int row0, int row1) {
+	const int H = row1 - row0;
 	if (nth <= 0) { long c = sysconf(_SC_NPROCESSORS_ONLN); nth = c > 0 ? (int)c : 1; }
 	if (nth > H) nth = H > 0 ? H : 1;
 	pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nth);
@@ -225,7 +226,8 @@ static int run(const rctx *C, int mode, float *out, double *out64, const float *
 	}
 	for (int i = 0; i < nth; i++) {
 		jobs[i].C = C; jobs[i].mode = mode; jobs[i].out = out; jobs[i].out64 = out64; jobs[i].dl = dl;
-		jobs[i].y0 = (int)(((long long)H * i) / nth); jobs[i].y1 = (int)(((long long)H * (i + 1)) / nth);
+		jobs[i].y0 = row0 + (int)(((long long)H * i) / nth); jobs[i].y1 = row0 + (int)(((long long)H * (i + 1)) / nth);
+		jobs[i].ybase = row0;
 		if (mode == 2) { jobs[i].ga = ga_part + (size_t)i * C->nmat * 3; jobs[i].ge = ge_part + (size_t)i * C->nlights * 3; }
 		pthread_create(&th[i], NULL, render_worker, &jobs[i]);
 	}
@@ -242,10 +244,12 @@ static int run(const rctx *C, int mode, float *out, double *out64, const float *
 }
 
 int ora_render(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat, const ora_light *lights,
-               int nlights, const ora_render_cfg *cfg, float *out_rgb, double *out_rgb64, int nthreads) {
+               int nlights, const ora_render_cfg *cfg, float *out_rgb, double *out_rgb64, int row0, int row1,
+               int nthreads) {
 	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
+	if (row0 < 0 || row1 > cfg->height || row0 > row1) return 1;
 	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg};
-	return run(&C, out_rgb64 ? 1 : 0, out_rgb, out_rgb64, NULL, NULL, NULL, nthreads);
+	return run(&C, out_rgb64 ? 1 : 0, out_rgb, out_rgb64, NULL, NULL, NULL, nthreads, row0, row1);
 }
 
 int ora_backward(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
@@ -255,5 +259,5 @@ int ora_backward(const ora_octree *t, const int32_t *mat_id, const ora_material 
 	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg};
 	memset(g_albedo, 0, sizeof(double) * nmat * 3);
 	memset(g_emission, 0, sizeof(double) * nlights * 3);
-	return run(&C, 2, NULL, NULL, dl_dimage, g_albedo, g_emission, nthreads);
+	return run(&C, 2, NULL, NULL, dl_dimage, g_albedo, g_emission, nthreads, 0, cfg->height);
 }

@@ -47,10 +47,12 @@ typedef struct {
 	float cam[13];                  /* position, right, up, forward, focal (0.5 / tan(fov/2)) */
 } ora_render_cfg;
 
-/* Pixel colours (row-major, H x W x 3) of one iteration.  fp64 != 0 accumulates
- * all shading in double (geometry and sampling stay fp32, so paths are identical). */
+/* Pixel colours (row-major, rows [row0, row1) x W x 3) of one iteration, into out_rgb
+ * (fp32 shading) or out_rgb64 (shading accumulated in double; geometry and sampling
+ * stay fp32, so the paths are identical). */
 int ora_render(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat, const ora_light *lights,
-               int nlights, const ora_render_cfg *cfg, float *out_rgb, double *out_rgb64, int nthreads);
+               int nlights, const ora_render_cfg *cfg, float *out_rgb, double *out_rgb64, int row0, int row1,
+               int nthreads);
 /* FP64 adjoint of the same estimator: given dL/dimage (H x W x 3), accumulates
  * dL/dalbedo (nmat x 3, Lambert albedo and mirror reflectance) and
  * dL/demission (nlights x 3).  Paths are replayed from their Philox keys. */
