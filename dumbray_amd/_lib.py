@@ -38,6 +38,10 @@ def _declare(L):
         "dr_scene_set_materials": (c_int, [vp, vp, c_int]),
         "dr_render_blocks": (c_int, [vp, vp, vp, c_int, c_int, vp]),
         "dr_backward": (c_int, [vp, vp, vp, vp, vp, vp]),
+        "dr_batch_create": (c_int, [vp, c_int, vp, vp]),
+        "dr_batch_run": (c_int, [vp, vp]),
+        "dr_batch_get_tuning": (c_int, [vp, vp]),
+        "dr_batch_destroy": (None, [vp]),
     }
     for name, (res, args) in decl.items():
         f = getattr(L, name)

@@ -11,6 +11,10 @@
 #include "dr_common.h"
 #include "dr_shade.h"
 
+#include <algorithm>
+#include <new>
+#include <vector>
+
 namespace dr {
 
 constexpr int kRB = 256;               // lanes per workgroup
@@ -100,26 +104,33 @@ __device__ __forceinline__ void reflect_factor(const dr_render_params &P, const 
 }
 
 // ------------------------------------------------------------------ forward
-__global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
-                                                 float *__restrict__ frame, int start_block, int levels) {
-  extern __shared__ uint32_t lds[];
-  __shared__ float acc[kBlockPixels * 3];
-  __shared__ int next_job;
-  uint32_t *stk = lds + threadIdx.x;
-  float *samp_col = (float *)(lds + levels * kRB);
-  const int block = start_block + blockIdx.x;
+// LDS used by one block render.
+struct BlockLds {
+  uint32_t *stk;        // this lane's first stack word ([level][lane])
+  float *samp_col;      // kBlockPixels * kRoundSamples * 3 sample colours
+  float *acc;           // kBlockPixels * 3 pixel sums
+  int *next_job;
+};
+
+// Renders framebuffer block `block` of one request (all lanes of the workgroup call it).
+// refill_min: an idle lane takes a new job once at least this many lanes of its wave
+// are idle (or the whole wave is).
+__device__ __forceinline__ void render_block(const DevTree &T, const float4 *__restrict__ shade,
+                                             const dr_render_params &P, float *__restrict__ frame, int block,
+                                             const BlockLds &lds, int refill_min) {
   const int spp = P.fsaa_x * P.fsaa_y;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
   const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
   const bool clip = P.cull_backfaces != 0;
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int t = threadIdx.x; t < kBlockPixels * 3; t += kRB) acc[t] = 0.0f;
+  float *samp_col = lds.samp_col;
+  for (int t = threadIdx.x; t < kBlockPixels * 3; t += kRB) lds.acc[t] = 0.0f;
 
   for (int s_base = 0; s_base < spp; s_base += kRoundSamples) {
     const int ns = min(kRoundSamples, spp - s_base);
     const int njobs = kBlockPixels * ns;       // job j: local pixel j / ns, sample s_base + j % ns
-    if (threadIdx.x == 0) next_job = 0;
+    if (threadIdx.x == 0) *lds.next_job = 0;
     __syncthreads();
     Path st;
     bool exhausted = false;
@@ -141,10 +152,11 @@ __global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restr
     for (;;) {
       const bool want = (st.job < 0) && !exhausted;
       const uint64_t wmask = __ballot(want);
-      if (wmask) {
+      const uint64_t busy = __ballot(st.job >= 0);
+      if (wmask && (__popcll(wmask) >= refill_min || busy == 0)) {
         const int leader = __ffsll((unsigned long long)wmask) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(&next_job, __popcll(wmask));
+        if (lane == leader) base = atomicAdd(lds.next_job, __popcll(wmask));
         base = __shfl(base, leader);
         if (want) {
           const int slot = base + __popcll(wmask & below);
@@ -163,7 +175,7 @@ __global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restr
       }
       if (st.job >= 0) {
         Hit hh;
-        const bool hit = tree_cast(T, st.o, st.d, clip, st.restrict_tri, stk, kRB, hh);
+        const bool hit = tree_cast(T, st.o, st.d, clip, st.restrict_tri, lds.stk, kRB, hh);
         if (st.phase == 0) {
           if (!hit) finish();
           else {
@@ -181,7 +193,7 @@ __global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restr
     __syncthreads();
     if (threadIdx.x < kBlockPixels) {
       for (int s = 0; s < ns; s++)
-        for (int c = 0; c < 3; c++) acc[3 * threadIdx.x + c] += samp_col[3 * (threadIdx.x * ns + s) + c] * spw;
+        for (int c = 0; c < 3; c++) lds.acc[3 * threadIdx.x + c] += samp_col[3 * (threadIdx.x * ns + s) + c] * spw;
     }
     __syncthreads();
   }
@@ -189,7 +201,8 @@ __global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restr
     int x, y;
     if (block_pixel(P.width, P.height, block, threadIdx.x, x, y)) {
       float4 *f = (float4 *)frame + (size_t)kBlockPixels * block + threadIdx.x;
-      const float4 col = make_float4(acc[3 * threadIdx.x], acc[3 * threadIdx.x + 1], acc[3 * threadIdx.x + 2], 1.0f);
+      const float *a = lds.acc + 3 * threadIdx.x;
+      const float4 col = make_float4(a[0], a[1], a[2], 1.0f);
       const float amount = (P.iteration <= 1) ? 1.0f : (1.0f / ((float)P.iteration));
       if (amount >= 1.0f) *f = col;
       else {
@@ -200,6 +213,16 @@ __global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restr
       }
     }
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
+                                                 float *__restrict__ frame, int start_block, int levels) {
+  extern __shared__ uint32_t lds[];
+  __shared__ float acc[kBlockPixels * 3];
+  __shared__ int next_job;
+  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kRB), acc, &next_job};
+  render_block(T, shade, P, frame, start_block + blockIdx.x, L, 1);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -333,6 +356,38 @@ __global__ __launch_bounds__(kRB) void k_backward(DevTree T, const float4 *__res
   }
 }
 
+// ------------------------------------------------------------------ batched requests
+struct DevRequest {
+  DevTree T;
+  const float4 *shade;
+  dr_render_params P;
+  float *frame;
+};
+
+__global__ void k_zero(int *counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+}
+
+// Persistent workgroups drain the queue of (request, first block, block count) items.
+__global__ __launch_bounds__(kRB) void k_batch(const DevRequest *__restrict__ reqs, const int4 *__restrict__ items,
+                                                int nitems, int *__restrict__ counter, int levels, int refill_min) {
+  extern __shared__ uint32_t lds[];
+  __shared__ float acc[kBlockPixels * 3];
+  __shared__ int next_job;
+  __shared__ int item;
+  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kRB), acc, &next_job};
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const int4 w = items[it];
+    const DevRequest &R = reqs[w.x];
+    for (int b = w.y; b < w.y + w.z; b++) render_block(R.T, R.shade, R.P, R.frame, b, L, refill_min);
+  }
+}
+
 static int check_params(dr_scene s, const dr_render_params *p) {
   if (!s || !p) return fail(DR_ERR_ARG, "render: null scene or parameters");
   if (!s->has_materials) return fail(DR_ERR_ARG, "render: dr_scene_set_materials has not been called");
@@ -380,4 +435,199 @@ extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *d
                      (const float4 *)s->shade.p, *p, dl_dframe, g_albedo, g_emission, levels, maxv);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
+}
+
+// ------------------------------------------------------------------ batch host side
+struct dr_batch_s {
+  int n = 0, nitems = 0, levels = 1, num_cu = 1;
+  dr_batch_tuning tuning{1, 2, 1};
+  dr::DevBuf reqs, items, counter;
+  std::vector<dr::DevRequest> host_reqs;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace dr {
+
+static size_t batch_lds(int levels) {
+  return ((size_t)levels * kRB + (size_t)kBlockPixels * kRoundSamples * 3) * 4;
+}
+
+// Queue of (request, first block, count), heaviest per-block cost first.
+static std::vector<int4> make_items(const std::vector<DevRequest> &R, int tile) {
+  std::vector<std::pair<double, int4>> v;
+  for (int r = 0; r < (int)R.size(); r++) {
+    const dr_render_params &P = R[r].P;
+    const int nb = block_count(&P);
+    const double cost = (double)P.fsaa_x * P.fsaa_y * (1.0 + P.max_bounces) * (1.0 + P.nlights);
+    for (int b = 0; b < nb; b += tile) v.push_back({cost * std::min(tile, nb - b), make_int4(r, b, std::min(tile, nb - b), 0)});
+  }
+  std::stable_sort(v.begin(), v.end(), [](const std::pair<double, int4> &a, const std::pair<double, int4> &b) {
+    return a.first > b.first;
+  });
+  std::vector<int4> out;
+  out.reserve(v.size());
+  for (auto &e : v) out.push_back(e.second);
+  return out;
+}
+
+static int launch_batch(const DevRequest *reqs_dev, const int4 *items_dev, int nitems, int *counter, int levels,
+                        const dr_batch_tuning &t, int num_cu, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, counter);
+  hipLaunchKernelGGL(k_batch, dim3(num_cu * t.groups_per_cu), dim3(kRB), batch_lds(levels), stream, reqs_dev,
+                     items_dev, nitems, counter, levels, t.refill_min);
+  DR_HIP_CHECK(hipGetLastError());
+  return DR_OK;
+}
+
+// Times one tuning on the proxy (every request at 1x1 spp, iteration 1, scratch frames).
+static int time_tuning(const std::vector<DevRequest> &proxy, const dr_batch_tuning &t, int levels, int num_cu,
+                       hipStream_t stream, DevBuf &reqs_dev, DevBuf &items_dev, int *counter, float &ms) {
+  const std::vector<int4> items = make_items(proxy, t.tile_blocks);
+  dev_free(items_dev);
+  int rc = dev_upload(items_dev, items.data(), items.size() * sizeof(int4));
+  if (rc != DR_OK) return rc;
+  hipEvent_t e0, e1;
+  DR_HIP_CHECK(hipEventCreate(&e0));
+  DR_HIP_CHECK(hipEventCreate(&e1));
+  rc = launch_batch((const DevRequest *)reqs_dev.p, (const int4 *)items_dev.p, (int)items.size(), counter, levels, t,
+                    num_cu, stream);
+  if (rc == DR_OK) {
+    DR_HIP_CHECK(hipEventRecord(e0, stream));
+    for (int k = 0; k < 2 && rc == DR_OK; k++)
+      rc = launch_batch((const DevRequest *)reqs_dev.p, (const int4 *)items_dev.p, (int)items.size(), counter,
+                        levels, t, num_cu, stream);
+    DR_HIP_CHECK(hipEventRecord(e1, stream));
+    DR_HIP_CHECK(hipEventSynchronize(e1));
+    DR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+static int autotune(const std::vector<DevRequest> &R, int levels, int num_cu, hipStream_t stream,
+                    dr_batch_tuning &best) {
+  std::vector<DevRequest> proxy = R;
+  std::vector<DevBuf> frames(R.size());
+  int rc = DR_OK;
+  for (size_t r = 0; r < R.size() && rc == DR_OK; r++) {
+    proxy[r].P.fsaa_x = proxy[r].P.fsaa_y = 1;
+    proxy[r].P.iteration = 1;
+    frames[r].bytes = (size_t)block_count(&R[r].P) * kBlockPixels * 16;
+    hipError_t e = hipMalloc(&frames[r].p, frames[r].bytes);
+    if (e != hipSuccess) { frames[r].p = nullptr; rc = fail(DR_ERR_ALLOC, "dr_batch_create: scratch frames"); }
+    proxy[r].frame = (float *)frames[r].p;
+  }
+  DevBuf reqs_dev, items_dev, counter;
+  if (rc == DR_OK) rc = dev_upload(reqs_dev, proxy.data(), proxy.size() * sizeof(DevRequest));
+  int zero = 0;
+  if (rc == DR_OK) rc = dev_upload(counter, &zero, sizeof(int));
+  best = dr_batch_tuning{1, 2, 1};
+  float best_ms = 0.0f;
+  if (rc == DR_OK) rc = time_tuning(proxy, best, levels, num_cu, stream, reqs_dev, items_dev, (int *)counter.p, best_ms);
+  // coordinate descent: tile, then persistent groups per CU, then refill cadence
+  const int tiles[] = {1, 2, 4, 8}, groups[] = {1, 2, 3, 4}, refills[] = {1, 8, 32};
+  for (int axis = 0; axis < 3 && rc == DR_OK; axis++) {
+    const int *vals = axis == 0 ? tiles : (axis == 1 ? groups : refills);
+    const int nv = axis == 2 ? 3 : 4;
+    dr_batch_tuning base = best;
+    for (int i = 0; i < nv && rc == DR_OK; i++) {
+      dr_batch_tuning t = base;
+      int &field = axis == 0 ? t.tile_blocks : (axis == 1 ? t.groups_per_cu : t.refill_min);
+      if (field == vals[i]) continue;
+      field = vals[i];
+      float ms = 0.0f;
+      rc = time_tuning(proxy, t, levels, num_cu, stream, reqs_dev, items_dev, (int *)counter.p, ms);
+      if (rc == DR_OK && ms < best_ms) { best_ms = ms; best = t; }
+    }
+  }
+  dev_free(reqs_dev);
+  dev_free(items_dev);
+  dev_free(counter);
+  for (auto &f : frames) dev_free(f);
+  return rc;
+}
+
+}  // namespace dr
+
+extern "C" int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tuning *tuning, dr_batch *out) {
+  if (!reqs || n < 1 || !out) return dr::fail(DR_ERR_ARG, "dr_batch_create: bad arguments");
+  *out = nullptr;
+  dr_batch b = new (std::nothrow) dr_batch_s();
+  if (!b) return dr::fail(DR_ERR_ALLOC, "dr_batch_create: out of host memory");
+  int rc = DR_OK;
+  for (int r = 0; r < n && rc == DR_OK; r++) {
+    rc = dr::check_params(reqs[r].scene, &reqs[r].params);
+    if (rc == DR_OK && !reqs[r].frame) rc = dr::fail(DR_ERR_ARG, "dr_batch_create: null frame");
+    if (rc != DR_OK) break;
+    dr::DevRequest d;
+    d.T = reqs[r].scene->tree;
+    d.shade = (const float4 *)reqs[r].scene->shade.p;
+    d.P = reqs[r].params;
+    d.frame = reqs[r].frame;
+    b->host_reqs.push_back(d);
+    b->levels = std::max(b->levels, reqs[r].scene->max_depth);
+  }
+  if (rc == DR_OK) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&b->num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) b->num_cu = 1;
+    b->n = n;
+    if (hipStreamCreate(&b->cap_stream) != hipSuccess) rc = dr::fail(DR_ERR_HIP, "dr_batch_create: stream");
+  }
+  if (rc == DR_OK) {
+    if (tuning) b->tuning = *tuning;
+    else rc = dr::autotune(b->host_reqs, b->levels, b->num_cu, b->cap_stream, b->tuning);
+  }
+  if (rc == DR_OK && (b->tuning.tile_blocks < 1 || b->tuning.groups_per_cu < 1 || b->tuning.refill_min < 1))
+    rc = dr::fail(DR_ERR_ARG, "dr_batch_create: tuning values must be >= 1");
+  if (rc == DR_OK) {
+    const std::vector<int4> items = dr::make_items(b->host_reqs, b->tuning.tile_blocks);
+    b->nitems = (int)items.size();
+    int zero = 0;
+    rc = dr::dev_upload(b->reqs, b->host_reqs.data(), b->host_reqs.size() * sizeof(dr::DevRequest));
+    if (rc == DR_OK) rc = dr::dev_upload(b->items, items.data(), items.size() * sizeof(int4));
+    if (rc == DR_OK) rc = dr::dev_upload(b->counter, &zero, sizeof(int));
+  }
+  if (rc == DR_OK) {
+    hipError_t e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      rc = dr::launch_batch((const dr::DevRequest *)b->reqs.p, (const int4 *)b->items.p, b->nitems, (int *)b->counter.p,
+                            b->levels, b->tuning, b->num_cu, b->cap_stream);
+      hipError_t e2 = hipStreamEndCapture(b->cap_stream, &b->graph);
+      if (rc == DR_OK && e2 != hipSuccess) rc = dr::fail(DR_ERR_HIP, "dr_batch_create: graph capture");
+      if (rc == DR_OK && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = dr::fail(DR_ERR_HIP, "dr_batch_create: graph instantiation");
+    } else {
+      rc = dr::fail(DR_ERR_HIP, std::string("dr_batch_create: capture: ") + hipGetErrorString(e));
+    }
+  }
+  if (rc != DR_OK) { dr_batch_destroy(b); return rc; }
+  *out = b;
+  return DR_OK;
+}
+
+extern "C" int dr_batch_run(dr_batch b, void *stream) {
+  if (!b || !b->exec) return dr::fail(DR_ERR_ARG, "dr_batch_run: bad batch");
+  DR_HIP_CHECK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return DR_OK;
+}
+
+extern "C" int dr_batch_get_tuning(dr_batch b, dr_batch_tuning *out) {
+  if (!b || !out) return dr::fail(DR_ERR_ARG, "dr_batch_get_tuning: bad arguments");
+  *out = b->tuning;
+  return DR_OK;
+}
+
+extern "C" void dr_batch_destroy(dr_batch b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  dr::dev_free(b->reqs);
+  dr::dev_free(b->items);
+  dr::dev_free(b->counter);
+  delete b;
 }

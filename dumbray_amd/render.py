@@ -125,6 +125,7 @@ class DumbRenderer:
         return self.light_table[: self.nlights, 9:12]
 
     def params(self, iteration: int = 1, request: int = 0) -> RenderParams:
+        """dr_render_params of this renderer (request = Philox counter word 3)."""
         p = RenderParams()
         p.width, p.height, p.fsaa_x, p.fsaa_y = self.width, self.height, self.fsaa_x, self.fsaa_y
         p.max_bounces, p.rr_start, p.rr_survive = self.max_bounces, self.rr_start, self.rr_survive
@@ -171,3 +172,56 @@ class DumbRenderer:
         check(lib().dr_backward(self.scene.handle, ctypes.byref(p), _vp(dl.data_ptr()), _vp(ga.data_ptr()),
                                 _vp(ge.data_ptr()), _vp(s.cuda_stream)), "dr_backward")
         return ga, ge[: self.nlights]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("params", RenderParams), ("frame", ctypes.c_void_p)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("tile_blocks", ctypes.c_int32), ("groups_per_cu", ctypes.c_int32), ("refill_min", ctypes.c_int32)]
+
+
+class RenderBatch:
+    """Many render requests in one persistent launch captured in a hipGraph (dr_batch_*).
+
+    requests: list of (DumbRenderer, BlockBasedFrameBuffer, iteration, request_id).  The
+    renderers' parameters are read when the batch is created.  tuning: None runs the
+    autotuner, else a dict with tile_blocks, groups_per_cu, refill_min."""
+
+    def __init__(self, requests, tuning=None):
+        self._keep = list(requests)
+        arr = (Request * len(requests))()
+        for i, (r, frame, iteration, req_id) in enumerate(requests):
+            if frame.getSize() != (r.width, r.height):
+                raise DumbRayError("RenderBatch: frame size differs from its renderer's")
+            arr[i].scene = r.scene.handle.value
+            arr[i].params = r.params(iteration, req_id)
+            arr[i].frame = frame.data.data_ptr()
+        t = None
+        if tuning is not None:
+            t = BatchTuning(int(tuning["tile_blocks"]), int(tuning["groups_per_cu"]), int(tuning["refill_min"]))
+        h = ctypes.c_void_p()
+        check(lib().dr_batch_create(arr, len(requests), ctypes.byref(t) if t is not None else None, ctypes.byref(h)),
+              "dr_batch_create")
+        self._h = h
+
+    @property
+    def tuning(self):
+        t = BatchTuning()
+        check(lib().dr_batch_get_tuning(self._h, ctypes.byref(t)), "dr_batch_get_tuning")
+        return {"tile_blocks": t.tile_blocks, "groups_per_cu": t.groups_per_cu, "refill_min": t.refill_min}
+
+    def run(self, stream=None):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(lib().dr_batch_run(self._h, _vp(s.cuda_stream)), "dr_batch_run")
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                lib().dr_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

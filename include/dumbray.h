@@ -134,6 +134,38 @@ int dr_render_blocks(dr_scene s, const dr_render_params *p, float *frame, int st
 int dr_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo, double *g_emission,
                 void *stream);
 
+/* ---------------------------------------------------------------- batched requests */
+
+/* One render request of a batch: a scene, its parameters (device material/light
+ * tables) and its device BlockBasedFrameBuffer<8,16> (params.width x params.height). */
+typedef struct {
+  dr_scene scene;
+  dr_render_params params;
+  float *frame;
+} dr_request;
+
+/* Launch shape of the batched kernel. */
+typedef struct {
+  int32_t tile_blocks;     /* framebuffer blocks per work item of the device queue */
+  int32_t groups_per_cu;   /* persistent workgroups per compute unit draining the queue */
+  int32_t refill_min;      /* idle lanes a wave collects before it takes new path jobs */
+} dr_batch_tuning;
+
+typedef struct dr_batch_s *dr_batch;
+
+/* Many renders in one persistent launch (replaces BlockRenderer's per-device block
+ * batches and host round trips, BlockRenderer.cu:74-97, FrameBuffer.impl.cuh:261-268,
+ * 355-379).  Work items (request, tile of blocks) are queued longest first and
+ * drained by persistent workgroups through one device atomic; the launch (queue
+ * reset + kernel) is captured once in a hipGraph.  tuning == NULL runs the autotuner
+ * (timed proxy renders into scratch frames; request frames are not touched).
+ * Every request's frame equals what dr_render_blocks would produce for it. */
+int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tuning *tuning, dr_batch *out);
+/* Replays the captured graph on `stream`. */
+int dr_batch_run(dr_batch b, void *stream);
+int dr_batch_get_tuning(dr_batch b, dr_batch_tuning *out);
+void dr_batch_destroy(dr_batch b);
+
 /* dr_cast plus per-ray work counters of the traversal that ran (measurement only;
  * no counterpart in the reference, which prints timings only): out_counts is a
  * device int32[n*4] receiving, per ray, the groups expanded (8 child box tests,
