@@ -111,6 +111,7 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
   T.root_is_leaf = flat.root_is_leaf;
   T.root_start = flat.root_start;
   T.root_count = flat.root_count;
+  T.margin = flat.prune_margin;
   s->max_depth = flat.max_depth;
   s->ntris = (int)t->host.tris.size();
   s->shade_host.resize(s->ntris);

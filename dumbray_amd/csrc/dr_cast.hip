@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
     }
     // ---- advance to the next non-empty leaf in DFS order
     if (r >= 0 && !leaf) {
-      if (next_leaf(T.groups, ray, stk, kCastBlock, sp, lg, lk, llink, lcnt, cc)) leaf = true;
+      if (next_leaf(T.groups, T.margin, ray, stk, kCastBlock, sp, lg, lk, llink, lcnt, cc)) leaf = true;
       else { write_result(r, false, h, out_id, out_t, out_p); flush_stats(r); r = -1; }
     }
     // ---- castInLeaf

@@ -22,20 +22,22 @@ struct OctreeHost {
 int build_octree(const float *tris9, int ntris, int split_threshold, int max_depth, int nthreads, OctreeHost &out);
 
 // ---------------------------------------------------------------- device layout
-// A "group" is the 8 contiguous children of one interior node, stored as 128
-// 32-bit words (512 B, four aligned 128-B lines):
-//   [0..47]    reference boxes: min.x[8] min.y[8] min.z[8] max.x[8] max.y[8] max.z[8]
-//   [48..95]   subtree leaf bounds, same order: the union of the boxes of the
-//              non-empty leaves below the child (inverted +inf/-inf box if none)
-//   [96..103]  link: interior -> child group index; leaf -> first triangle record
-//   [104..111] count: -1 interior, 0 empty leaf (or only degenerate triangles), >0 records
-//   [112..127] unused
+// A "group" is the 8 contiguous children of one interior node, stored as 72
+// 32-bit words (288 B):
+//   [0..47]  reference boxes: min.x[8] min.y[8] min.z[8] max.x[8] max.y[8] max.z[8]
+//   [48]     bit k set iff child k has a non-empty leaf at or below it
+//   [49..55] unused
+//   [56..63] link: interior -> child group index; leaf -> first triangle record
+//   [64..71] count: -1 interior, 0 empty leaf (or only degenerate triangles), >0 records
 // Child k uses the reference's numbering (bit0 = z-high, bit1 = y-high, bit2 = x-high).
-// The subtree leaf bounds only prune work that cannot change the result: a
-// leaf accepts a hit only if the hit point o + d*t (t >= 0) lies in its box,
-// and that point never leaves the closed octant of o that d points into.
-constexpr int kGroupWords = 128;
+// Traversal drops a child whose box, widened by the tree's prune margin, lies
+// outside the ray's forward octant: a leaf accepts a hit only if the hit point
+// o + d*t (t >= 0) is inside its box, that point never leaves the closed octant
+// of o that d points into, and every non-empty leaf box lies inside each of its
+// ancestors' boxes widened by the margin (computed exactly when flattening).
+constexpr int kGroupWords = 72;
 constexpr int kGroupF4 = kGroupWords / 4;
+constexpr int kGroupMask = 48, kGroupLink = 56, kGroupCount = 64;
 // A triangle record is 16 words (64 B):
 //   a.x a.y a.z n.x | n.y n.z b.x b.y | b.z c.x c.y c.z | thr0 thr1 thr2 id
 // n = (b-a) x (c-a) and thr* = the ray-independent right-hand sides of the
@@ -49,6 +51,7 @@ struct OctreeFlat {
   Box root;
   int root_is_leaf = 0, root_start = 0, root_count = 0;
   int ngroups = 0, nrec = 0, max_depth = 0;
+  float prune_margin = 0.0f;               // see the group layout above
 };
 int flatten_octree(const OctreeHost &t, OctreeFlat &out);
 

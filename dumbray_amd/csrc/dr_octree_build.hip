@@ -16,6 +16,7 @@
 #include <mutex>
 #include <thread>
 #include <vector>
+#include <cmath>
 #include <cstring>
 
 namespace dr {
@@ -457,23 +458,37 @@ int flatten_octree(const OctreeHost &t, OctreeFlat &out) {
     }
     slb[i] = b;
   }
+  // prune margin: how far any non-empty leaf box reaches beyond the box of any of its
+  // ancestors (evaluated in double), plus a guard of 4 ulps of the largest coordinate so
+  // that the fp32 sum box + margin is never below the leaf's face.
+  double poke = 0.0, maxabs = 0.0;
+  for (int i = 1; i < nn; i++) {
+    const Box &b = t.bounds[i], &l = slb[i];
+    if (!(l.s.x <= l.e.x)) continue;                   // no non-empty leaf below
+    const double d[6] = {(double)l.e.x - b.e.x, (double)l.e.y - b.e.y, (double)l.e.z - b.e.z,
+                         (double)b.s.x - l.s.x, (double)b.s.y - l.s.y, (double)b.s.z - l.s.z};
+    for (double v : d) poke = std::max(poke, v);
+    const double c[6] = {b.s.x, b.s.y, b.s.z, b.e.x, b.e.y, b.e.z};
+    for (double v : c) maxabs = std::max(maxabs, std::fabs(v));
+  }
+  out.prune_margin = (float)(poke + 4.0 * maxabs * 1.1920928955078125e-07 + 1e-30);
   out.groups.assign((size_t)out.ngroups * kGroupWords, 0.0f);
   for (int g = 0; g < out.ngroups; g++) {
     float *G = &out.groups[(size_t)g * kGroupWords];
+    uint32_t nonempty = 0;
     for (int k = 0; k < 8; k++) {
       const int node = 1 + 8 * g + k;
       const Box &b = t.bounds[node];
-      const Box &l = slb[node];
       G[0 + k] = b.s.x; G[8 + k] = b.s.y; G[16 + k] = b.s.z;
       G[24 + k] = b.e.x; G[32 + k] = b.e.y; G[40 + k] = b.e.z;
-      G[48 + k] = l.s.x; G[56 + k] = l.s.y; G[64 + k] = l.s.z;
-      G[72 + k] = l.e.x; G[80 + k] = l.e.y; G[88 + k] = l.e.z;
+      if (slb[node].s.x <= slb[node].e.x) nonempty |= 1u << k;
       int32_t link, count;
       if (t.children[node] >= 0) { link = (t.children[node] - 1) / 8; count = -1; }
       else leaf_records(node, link, count);
-      memcpy(&G[96 + k], &link, 4);
-      memcpy(&G[104 + k], &count, 4);
+      memcpy(&G[kGroupLink + k], &link, 4);
+      memcpy(&G[kGroupCount + k], &count, 4);
     }
+    memcpy(&G[kGroupMask], &nonempty, 4);
   }
   out.nrec = (int)(out.recs.size() / kRecWords);
   return 0;

@@ -21,7 +21,7 @@ struct DevTree {
   const float4 *__restrict__ recs;     // 4 float4 per triangle record
   float rsx, rsy, rsz, rex, rey, rez;  // root box
   int root_is_leaf, root_start, root_count;
-  int pad;
+  float margin;                        // prune margin of the octant test (see the group layout)
 };
 
 struct Hit {
@@ -32,7 +32,7 @@ struct Hit {
 
 // Work counters of one cast (measurement launches only; nullptr elsewhere).
 struct CastCounters {
-  int expand;    // groups whose 8 child tests ran (512-B group reads)
+  int expand;    // groups whose 8 child tests ran (196-B reads: 8 boxes + non-empty mask)
   int leaves;    // leaves whose triangles were tested (+ leaf box and link/count reads)
   int plane;     // triangle records whose plane test ran (first 32 B)
   int inside;    // records that reached the inside test (second 32 B)
@@ -66,28 +66,51 @@ __device__ __forceinline__ bool inside(V3 p, V3 a, V3 b, V3 c, float thr0, float
 
 // castInLeaf: closest accepted triangle among `count` records; accepted iff its hit
 // point lies inside the leaf box.  exclude = validateNotSameAsObject's object (or -1).
+// Records are read DR_LEAF_CHUNK at a time so their loads are in flight together;
+// the candidates of a chunk are then resolved in record order, which keeps the
+// reference's strict '<' first-wins rule.
+#ifndef DR_LEAF_CHUNK
+#define DR_LEAF_CHUNK 4
+#endif
 __device__ __forceinline__ bool leaf_cast(const float4 *__restrict__ R, int start, int count, const Box &lb, V3 o,
                                           V3 d, bool clip, int exclude, Hit &h, CastCounters *cc = nullptr) {
+  constexpr int C = DR_LEAF_CHUNK;
   if (cc) { cc->leaves++; cc->plane += count; }
   float best = FLT_MAX;
   V3 bp = v3(0.f, 0.f, 0.f);
   int bid = -1;
-  for (int k = start; k < start + count; ++k) {
-    const float4 r0 = R[4 * k + 0], r1 = R[4 * k + 1];
-    const V3 a = v3(r0.x, r0.y, r0.z), n = v3(r0.w, r1.x, r1.y);
-    const float dp = dot(a - o, n);
-    if (clip && (dp < -kVecEps)) continue;
-    const float dd = dot(d, n);
-    if (dp * dd <= 0) continue;
-    const float t = dp / dd;
-    if (!(t < best)) continue;                       // could not replace the current best
-    if (cc) cc->inside++;
-    const float4 r2 = R[4 * k + 2], r3 = R[4 * k + 3];
-    const int id = __float_as_int(r3.w);
-    if (id == exclude) continue;
-    const V3 p = o + d * t;
-    const V3 b = v3(r1.z, r1.w, r2.x), c = v3(r2.y, r2.z, r2.w);
-    if (inside(p, a, b, c, r3.x, r3.y, r3.z)) { best = t; bp = p; bid = id; }
+  const int end = start + count;
+  for (int k0 = start; k0 < end; k0 += C) {
+    float4 r0[C], r1[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const int k = min(k0 + j, end - 1);
+      r0[j] = R[4 * k + 0];
+      r1[j] = R[4 * k + 1];
+    }
+    float tj[C];
+    bool cand[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      const V3 a = v3(r0[j].x, r0[j].y, r0[j].z), n = v3(r0[j].w, r1[j].x, r1[j].y);
+      const float dp = dot(a - o, n);
+      const float dd = dot(d, n);
+      tj[j] = dp / dd;
+      cand[j] = (k0 + j < end) && !(clip && (dp < -kVecEps)) && !(dp * dd <= 0) && (tj[j] < best);
+    }
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+      if (!cand[j] || !(tj[j] < best)) continue;                 // could not replace the current best
+      if (cc) cc->inside++;
+      const int k = k0 + j;
+      const float4 r2 = R[4 * k + 2], r3 = R[4 * k + 3];
+      const int id = __float_as_int(r3.w);
+      if (id == exclude) continue;
+      const V3 p = o + d * tj[j];
+      const V3 a = v3(r0[j].x, r0[j].y, r0[j].z);
+      const V3 b = v3(r1[j].z, r1[j].w, r2.x), c = v3(r2.y, r2.z, r2.w);
+      if (inside(p, a, b, c, r3.x, r3.y, r3.z)) { best = tj[j]; bp = p; bid = id; }
+    }
   }
   if (best != FLT_MAX && contains(lb, bp)) { h.id = bid; h.t = best; h.p = bp; return true; }
   return false;
@@ -103,32 +126,28 @@ __device__ __forceinline__ Box child_box(const float4 *__restrict__ G, int g, in
 }
 
 // 8 child tests of group g -> mask in visiting order (bit i = child canon ^ i):
-// reference slab test on the child's box, child has a non-empty leaf below it, and
-// that leaf's box can be reached by the forward ray (octant test on subtree bounds).
-__device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int g, const TRay &r) {
+// the child has a non-empty leaf below it, its box widened by the prune margin meets
+// the forward octant of the ray, and the reference slab test on its box passes.
+__device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int g, const TRay &r, float M) {
   const float4 *q = G + kGroupF4 * (size_t)g;
   uint32_t m = 0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {       // two halves of 4 children keep register use low
     const float4 sx = q[0 + h], sy = q[2 + h], sz = q[4 + h], ex = q[6 + h], ey = q[8 + h], ez = q[10 + h];
-    const float4 lsx = q[12 + h], lsy = q[14 + h], lsz = q[16 + h], lex = q[18 + h], ley = q[20 + h],
-                 lez = q[22 + h];
     const float a_sx[4] = {sx.x, sx.y, sx.z, sx.w}, a_sy[4] = {sy.x, sy.y, sy.z, sy.w},
                 a_sz[4] = {sz.x, sz.y, sz.z, sz.w}, a_ex[4] = {ex.x, ex.y, ex.z, ex.w},
                 a_ey[4] = {ey.x, ey.y, ey.z, ey.w}, a_ez[4] = {ez.x, ez.y, ez.z, ez.w};
-    const float b_sx[4] = {lsx.x, lsx.y, lsx.z, lsx.w}, b_sy[4] = {lsy.x, lsy.y, lsy.z, lsy.w},
-                b_sz[4] = {lsz.x, lsz.y, lsz.z, lsz.w}, b_ex[4] = {lex.x, lex.y, lex.z, lex.w},
-                b_ey[4] = {ley.x, ley.y, ley.z, ley.w}, b_ez[4] = {lez.x, lez.y, lez.z, lez.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       // forward octant of o: coordinate k of o + d*t (t >= 0) is >= o.k if d.k >= 0, <= o.k if d.k <= 0
-      const bool behind = (r.d.x >= 0 && b_ex[j] < r.o.x) || (r.d.x <= 0 && b_sx[j] > r.o.x) ||
-                          (r.d.y >= 0 && b_ey[j] < r.o.y) || (r.d.y <= 0 && b_sy[j] > r.o.y) ||
-                          (r.d.z >= 0 && b_ez[j] < r.o.z) || (r.d.z <= 0 && b_sz[j] > r.o.z);
+      const bool behind = (r.d.x >= 0 && a_ex[j] + M < r.o.x) || (r.d.x <= 0 && a_sx[j] - M > r.o.x) ||
+                          (r.d.y >= 0 && a_ey[j] + M < r.o.y) || (r.d.y <= 0 && a_sy[j] - M > r.o.y) ||
+                          (r.d.z >= 0 && a_ez[j] + M < r.o.z) || (r.d.z <= 0 && a_sz[j] - M > r.o.z);
       const bool hit = !behind && slab_hit(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv);
       m |= hit ? (1u << (4 * h + j)) : 0u;
     }
   }
+  m &= ((const uint32_t *)q)[kGroupMask];
   const int c = r.canon;
   if (c & 1) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
   if (c & 2) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
@@ -139,8 +158,8 @@ __device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int
 // Pops the next child to visit.  Returns 1 with (g, k, link, cnt) set when a
 // non-empty leaf was reached, pushes interior children, returns 0 when the
 // stack is exhausted (miss).
-__device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, const TRay &r, uint32_t *stk, int stride,
-                                         int &sp, int &g_out, int &k_out, int &link_out, int &cnt_out,
+__device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, float M, const TRay &r, uint32_t *stk,
+                                         int stride, int &sp, int &g_out, int &k_out, int &link_out, int &cnt_out,
                                          CastCounters *cc = nullptr) {
   const int *Gi = (const int *)G;
   while (sp >= 0) {
@@ -152,11 +171,11 @@ __device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, const TRa
     const int g = (int)(f >> 8);
     const int k = r.canon ^ i;
     const size_t base = kGroupWords * (size_t)g;
-    const int link = Gi[base + 96 + k];
-    const int cnt = Gi[base + 104 + k];
+    const int link = Gi[base + kGroupLink + k];
+    const int cnt = Gi[base + kGroupCount + k];
     if (cnt < 0) {
       ++sp;
-      stk[sp * stride] = ((uint32_t)link << 8) | group_mask(G, link, r);
+      stk[sp * stride] = ((uint32_t)link << 8) | group_mask(G, link, r, M);
       if (cc) cc->expand++;
     } else {
       g_out = g; k_out = k; link_out = link; cnt_out = cnt;
@@ -172,7 +191,7 @@ __device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint3
   if (T.root_is_leaf) return 2;
   if (!slab_hit(T.rsx, T.rsy, T.rsz, T.rex, T.rey, T.rez, r.o, r.inv)) return 0;
   sp = 0;
-  stk[0] = group_mask(T.groups, 0, r);               // group 0 = the root's children
+  stk[0] = group_mask(T.groups, 0, r, T.margin);     // group 0 = the root's children
   if (cc) cc->expand++;
   return 1;
 }
@@ -190,7 +209,7 @@ __device__ __forceinline__ bool tree_cast(const DevTree &T, V3 o, V3 d, bool cli
     return leaf_cast(T.recs, T.root_start, T.root_count, rb, o, d, clip, exclude, h);
   }
   int g, k, link, cnt;
-  while (next_leaf(T.groups, r, stk, stride, sp, g, k, link, cnt))
+  while (next_leaf(T.groups, T.margin, r, stk, stride, sp, g, k, link, cnt))
     if (leaf_cast(T.recs, link, cnt, child_box(T.groups, g, k), o, d, clip, exclude, h)) return true;
   return false;
 }

@@ -169,7 +169,7 @@ void dr_batch_destroy(dr_batch b);
 /* dr_cast plus per-ray work counters of the traversal that ran (measurement only;
  * no counterpart in the reference, which prints timings only): out_counts is a
  * device int32[n*4] receiving, per ray, the groups expanded (8 child box tests,
- * 512 B read each), leaves tested, triangle plane tests (32 B record reads) and
+ * 196 B read each), leaves tested, triangle plane tests (32 B record reads) and
  * inside tests (a further 32 B each).  Same results as dr_cast. */
 int dr_cast_stats(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude, int32_t *out_id,
                   float *out_t, int32_t *out_counts, void *stream);
