@@ -102,6 +102,12 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
   (void)hipGetDevice(&s->device);
   int rc = dr::dev_upload(s->groups, flat.groups.data(), flat.groups.size() * sizeof(float));
   if (rc == DR_OK) rc = dr::dev_upload(s->recs, flat.recs.data(), flat.recs.size() * sizeof(float));
+  if (rc == DR_OK) {
+    const std::vector<uint32_t> zeros(2 * dr::kWorkSlots, 0u);
+    rc = dr::dev_upload(s->work, zeros.data(), zeros.size() * sizeof(uint32_t));
+  }
+  if (rc == DR_OK && hipDeviceGetAttribute(&s->num_cu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess)
+    rc = dr::fail(DR_ERR_HIP, "dr_scene_create: cannot query the compute unit count");
   if (rc != DR_OK) { dr_scene_destroy(s); return rc; }
   dr::DevTree &T = s->tree;
   T.groups = (const float4 *)s->groups.p;
@@ -145,6 +151,7 @@ void dr_scene_destroy(dr_scene s) {
   dr::dev_free(s->groups);
   dr::dev_free(s->recs);
   dr::dev_free(s->shade);
+  dr::dev_free(s->work);
   delete s;
 }
 

@@ -1,18 +1,19 @@
-// Batch closest-hit ray cast.
+// Batch closest-hit ray cast (Octree::cast over a ray batch, Octree.impl.cuh:174-212).
 //
-// Each 256-lane block owns a contiguous chunk of rays.  Lanes whose ray has
-// finished take the next ray of the chunk: the wave ballots its idle lanes,
-// one lane reserves that many rays with a single LDS atomic, and each idle lane
-// takes slot base + (number of idle lanes below it) (wave64 ballot + mbcnt
-// prefix).  Every loop iteration a lane either advances to its next non-empty
-// leaf or tests the triangles of the leaf it holds, so the triangle tests of a
-// wave run together instead of interleaving with node expansion.
+// A persistent grid of 256-lane blocks; each lane carries one ray and its own
+// DFS stack in LDS.  The loop advances every traversing lane by one bounded step
+// per iteration and batches the rarer, divergent work (leaf tests, taking new
+// rays) until enough lanes need it.
+#include <algorithm>
 #include "dr_common.h"
 
 namespace dr {
 
 constexpr int kCastBlock = 256;
-constexpr int kCastChunk = 2048;      // rays per block
+constexpr int kWavesPerBlock = kCastBlock / 64;
+constexpr int kWaveChunk = 128;       // rays a wave takes from the global counter at a time
+constexpr int kRefillMin = 16;        // idle lanes that trigger a refill
+constexpr int kLeafMin = 32;          // pending leaves that trigger the leaf tests
 
 __device__ __forceinline__ void write_result(int r, bool hit, const Hit &h, int *__restrict__ out_id,
                                              float *__restrict__ out_t, float *__restrict__ out_p) {
@@ -25,62 +26,77 @@ __device__ __forceinline__ void write_result(int r, bool hit, const Hit &h, int 
   }
 }
 
-// Dynamic LDS: `levels` stack words per lane, [level][lane].  kStats: also write
-// the per-ray work counters (4 x int32 per ray) to `stats`.
+// One bounded step per loop iteration.  A lane that is traversing pops one child:
+// an interior child is expanded and pushed, a leaf child is held as pending.
+// Pending leaves are tested together once at least kLeafMin lanes hold one (or
+// no lane is still traversing), and idle lanes take new rays once kRefillMin of
+// them are idle (or no lane is busy), so the node expansions, which dominate the
+// work, run with most lanes of the wave active.
+//
+// Rays are handed out dynamically: a wave owns a range of kWaveChunk rays taken
+// from the global counter work[0]; lanes that need a ray take the next ones of the
+// range, and the wave takes a new range when it runs out.  The last wave to exit
+// (counted in work[1]) sets both counters back to zero for the next launch.
 template <bool kStats>
 __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__restrict__ rays, int n, int clip,
                                                       const int *__restrict__ exclude, int *__restrict__ out_id,
                                                       float *__restrict__ out_t, float *__restrict__ out_p,
-                                                      int *__restrict__ stats) {
+                                                      int *__restrict__ stats, unsigned *__restrict__ work) {
   extern __shared__ uint32_t stk_base[];
-  __shared__ int next_ray;
   uint32_t *stk = stk_base + threadIdx.x;
-  const int chunk_end = min(n, (int)(blockIdx.x + 1) * kCastChunk);
-  if (threadIdx.x == 0) next_ray = blockIdx.x * kCastChunk;
-  __syncthreads();
+  constexpr int S = kCastBlock;
 
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  unsigned wcur = 0, wend = 0;          // the wave's current range of rays (wave-uniform)
   int r = -1;
   bool exhausted = false;
-  TRay ray;
-  int ex = -1, sp = -1;
-  bool leaf = false;
-  int lg = -1, lk = 0, llink = 0, lcnt = 0;
-  Hit h;
-  h.id = -1; h.t = FLT_MAX; h.p = v3(0.f, 0.f, 0.f);
+  Walk w;
+  w.state = kWalkDone;
   CastCounters cnt_local = {0, 0, 0, 0};
   CastCounters *cc = kStats ? &cnt_local : nullptr;
-  auto flush_stats = [&](int ray_index) {
+  auto finish = [&]() {
+    write_result(r, w.hit, w.h, out_id, out_t, out_p);
     if (kStats) {
-      stats[4 * (size_t)ray_index + 0] = cnt_local.expand;
-      stats[4 * (size_t)ray_index + 1] = cnt_local.leaves;
-      stats[4 * (size_t)ray_index + 2] = cnt_local.plane;
-      stats[4 * (size_t)ray_index + 3] = cnt_local.inside;
+      stats[4 * (size_t)r + 0] = cnt_local.expand;
+      stats[4 * (size_t)r + 1] = cnt_local.leaves;
+      stats[4 * (size_t)r + 2] = cnt_local.plane;
+      stats[4 * (size_t)r + 3] = cnt_local.inside;
       cnt_local = {0, 0, 0, 0};
     }
+    r = -1;
   };
 
   for (;;) {
-    // ---- refill idle lanes from the block's chunk
+    // ---- refill idle lanes
     const bool want = (r < 0) && !exhausted;
     const uint64_t wmask = __ballot(want);
-    if (wmask) {
-      const int leader = __ffsll((unsigned long long)wmask) - 1;
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&next_ray, __popcll(wmask));
-      base = __shfl(base, leader);
+    if (wmask && (__popcll(wmask) >= kRefillMin || __ballot(r >= 0) == 0)) {
+      const unsigned need = (unsigned)__popcll(wmask);
+      const unsigned idx = (unsigned)__popcll(wmask & below);
+      const unsigned rem = wend - wcur;
+      int slot;
+      if (rem >= need) {
+        slot = (int)(wcur + idx);
+        wcur += need;
+      } else {
+        const int leader = __ffsll((unsigned long long)wmask) - 1;
+        unsigned nb = 0;
+        if (lane == leader) nb = atomicAdd(work, (unsigned)kWaveChunk);
+        nb = __shfl(nb, leader);
+        const unsigned top = min(nb + (unsigned)kWaveChunk, (unsigned)n);
+        const unsigned len = nb < top ? top - nb : 0u;
+        slot = idx < rem ? (int)(wcur + idx) : (idx - rem < len ? (int)(nb + idx - rem) : -1);
+        wcur = nb + min(need - rem, len);
+        wend = nb + len;
+      }
       if (want) {
-        const int slot = base + __popcll(wmask & below);
-        if (slot < chunk_end) {
+        if (slot >= 0) {
           r = slot;
           const float2 *R = (const float2 *)(rays + 6 * (size_t)r);
           const float2 q0 = R[0], q1 = R[1], q2 = R[2];
-          ray = make_tray(v3(q0.x, q0.y, q1.x), v3(q1.y, q2.x, q2.y));
-          ex = exclude ? exclude[r] : -1;
-          const int st = tree_begin(T, ray, stk, sp, cc);
-          if (st == 0) { write_result(r, false, h, out_id, out_t, out_p); flush_stats(r); r = -1; }
-          else if (st == 2) { leaf = true; lg = -1; llink = T.root_start; lcnt = T.root_count; }
+          walk_begin(T, w, v3(q0.x, q0.y, q1.x), v3(q1.y, q2.x, q2.y), exclude ? exclude[r] : -1, stk, S, cc);
+          if (w.state == kWalkDone) finish();
         } else {
           exhausted = true;
         }
@@ -90,19 +106,27 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
       if (__ballot(!exhausted) == 0) break;
       continue;
     }
-    // ---- advance to the next non-empty leaf in DFS order
-    if (r >= 0 && !leaf) {
-      if (next_leaf(T.groups, T.margin, ray, stk, kCastBlock, sp, lg, lk, llink, lcnt, cc)) leaf = true;
-      else { write_result(r, false, h, out_id, out_t, out_p); flush_stats(r); r = -1; }
+    // ---- one traversal step
+    if (r >= 0 && w.state == kWalkTrace) {
+      walk_step(T, w, stk, S, cc);
+      if (w.state == kWalkDone) finish();
     }
-    // ---- castInLeaf
-    if (leaf) {
-      leaf = false;
-      Box lb;
-      if (lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
-      else lb = child_box(T.groups, lg, lk);
-      const bool hit = leaf_cast(T.recs, llink, lcnt, lb, ray.o, ray.d, clip != 0, ex, h, cc);
-      if (hit || lg < 0) { write_result(r, hit, h, out_id, out_t, out_p); flush_stats(r); r = -1; }
+    // ---- castInLeaf for the pending leaves
+    const uint64_t pm = __ballot(r >= 0 && w.state == kWalkLeaf);
+    if (pm && (__popcll(pm) >= kLeafMin || __ballot(r >= 0 && w.state == kWalkTrace) == 0)) {
+      if (r >= 0 && w.state == kWalkLeaf) {
+        walk_leaf(T, w, clip != 0, cc);
+        if (w.state == kWalkDone) finish();
+      }
+    }
+  }
+  // ---- the last wave out resets the counters
+  if (lane == 0) {
+    __threadfence();
+    const unsigned total = gridDim.x * (kCastBlock / 64);
+    if (atomicAdd(work + 1, 1u) == total - 1) {
+      atomicExch(work, 0u);
+      atomicExch(work + 1, 0u);
     }
   }
 }
@@ -112,15 +136,20 @@ static int launch_cast(dr_scene s, const float *rays, int n, unsigned flags, con
   if (s->max_depth > kMaxTreeDepth) return fail(DR_ERR_ARG, "dr_cast: octree deeper than the traversal stack");
   if (n == 0) return DR_OK;
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
-  const size_t lds = (size_t)levels * kCastBlock * sizeof(uint32_t);
-  const int blocks = (n + kCastChunk - 1) / kCastChunk;
   const int clip = (int)(flags & DR_CAST_CLIP_BACKFACES);
+  const size_t lds = (size_t)levels * kStackWordsPerLevel * kCastBlock * sizeof(uint32_t);
+  int per_cu = 0;
+  DR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cast<false>, kCastBlock, lds));
+  // persistent grid: at most one block per resident slot, fewer for small batches
+  const long long need = ((long long)n + kWavesPerBlock * kWaveChunk - 1) / (kWavesPerBlock * kWaveChunk);
+  const int blocks = (int)std::max(1LL, std::min<long long>(need, (long long)s->num_cu * std::max(per_cu, 1)));
+  unsigned *work = (unsigned *)s->work.p + 2 * (s->work_slot.fetch_add(1) % kWorkSlots);
   if (stats)
     hipLaunchKernelGGL(k_cast<true>, dim3(blocks), dim3(kCastBlock), lds, (hipStream_t)stream, s->tree, rays, n, clip,
-                       (const int *)exclude, (int *)out_id, out_t, out_point, (int *)stats);
+                       (const int *)exclude, (int *)out_id, out_t, out_point, (int *)stats, work);
   else
     hipLaunchKernelGGL(k_cast<false>, dim3(blocks), dim3(kCastBlock), lds, (hipStream_t)stream, s->tree, rays, n,
-                       clip, (const int *)exclude, (int *)out_id, out_t, out_point, (int *)nullptr);
+                       clip, (const int *)exclude, (int *)out_id, out_t, out_point, (int *)nullptr, work);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
 }

@@ -1,6 +1,7 @@
 // Internal helpers shared by the C-ABI translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <string>
 #include <vector>
 #include "../../include/dumbray.h"
@@ -8,6 +9,8 @@
 #include "dr_traverse.h"
 
 namespace dr {
+
+constexpr int kWorkSlots = 1024;         // cast work-counter pairs per scene
 
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
@@ -41,5 +44,11 @@ struct dr_scene_s {
   int max_mat = -1;                        // largest material id in use
   // per triangle: n = (b-a) x (c-a) and the material id (bits of .w)
   std::vector<float4> shade_host;
+  // Work counters of the cast kernels: kWorkSlots pairs (next ray, waves done), zero
+  // between launches; each launch takes the next pair, so launches in flight at the
+  // same time (up to kWorkSlots of them) use distinct counters.
+  dr::DevBuf work;
+  std::atomic<uint32_t> work_slot{0};
+  int num_cu = 1;
   int64_t bytes() const { return (int64_t)(groups.bytes + recs.bytes + shade.bytes); }
 };

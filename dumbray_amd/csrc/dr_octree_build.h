@@ -25,8 +25,12 @@ int build_octree(const float *tris9, int ntris, int split_threshold, int max_dep
 // A "group" is the 8 contiguous children of one interior node, stored as 72
 // 32-bit words (288 B):
 //   [0..47]  reference boxes: min.x[8] min.y[8] min.z[8] max.x[8] max.y[8] max.z[8]
-//   [48]     bit k set iff child k has a non-empty leaf at or below it
-//   [49..55] unused
+//   [48]     bits 0-7: child k has a non-empty leaf at or below it;
+//            bits 8-15: child k is interior
+//   [49]     group of the first interior child.  Interior children get consecutive
+//            groups (breadth-first order), so interior child k's group is this
+//            plus the number of interior children before k.
+//   [50..55] unused
 //   [56..63] link: interior -> child group index; leaf -> first triangle record
 //   [64..71] count: -1 interior, 0 empty leaf (or only degenerate triangles), >0 records
 // Child k uses the reference's numbering (bit0 = z-high, bit1 = y-high, bit2 = x-high).
@@ -37,7 +41,7 @@ int build_octree(const float *tris9, int ntris, int split_threshold, int max_dep
 // ancestors' boxes widened by the margin (computed exactly when flattening).
 constexpr int kGroupWords = 72;
 constexpr int kGroupF4 = kGroupWords / 4;
-constexpr int kGroupMask = 48, kGroupLink = 56, kGroupCount = 64;
+constexpr int kGroupMask = 48, kGroupBase = 49, kGroupLink = 56, kGroupCount = 64;
 // A triangle record is 16 words (64 B):
 //   a.x a.y a.z n.x | n.y n.z b.x b.y | b.z c.x c.y c.z | thr0 thr1 thr2 id
 // n = (b-a) x (c-a) and thr* = the ray-independent right-hand sides of the

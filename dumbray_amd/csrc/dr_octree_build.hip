@@ -370,7 +370,8 @@ int build_octree(const float *tris9, int ntris, int split_threshold, int max_dep
   if (B.failed) return -2;
   B.reduce_interior(root);
 
-  // Export in depth-first preorder so the node numbering is deterministic.
+  // Export in breadth-first order so the node numbering is deterministic and the
+  // interior children of a node own consecutive child blocks.
   const int32_t total = B.pool.size();
   out.bounds.clear();
   out.children.clear();
@@ -378,7 +379,7 @@ int build_octree(const float *tris9, int ntris, int split_threshold, int max_dep
   out.list.clear();
   out.bounds.reserve(total);
   std::vector<int32_t> remap(total, -1);
-  // iterative preorder over groups: node -> new index; children stay contiguous
+  // queue over groups: node -> new index; children stay contiguous
   std::vector<int32_t> order;
   order.reserve(total);
   order.push_back(root);
@@ -475,7 +476,8 @@ int flatten_octree(const OctreeHost &t, OctreeFlat &out) {
   out.groups.assign((size_t)out.ngroups * kGroupWords, 0.0f);
   for (int g = 0; g < out.ngroups; g++) {
     float *G = &out.groups[(size_t)g * kGroupWords];
-    uint32_t nonempty = 0;
+    uint32_t nonempty = 0, interior = 0;
+    int32_t first_child = 0;
     for (int k = 0; k < 8; k++) {
       const int node = 1 + 8 * g + k;
       const Box &b = t.bounds[node];
@@ -483,12 +485,22 @@ int flatten_octree(const OctreeHost &t, OctreeFlat &out) {
       G[24 + k] = b.e.x; G[32 + k] = b.e.y; G[40 + k] = b.e.z;
       if (slb[node].s.x <= slb[node].e.x) nonempty |= 1u << k;
       int32_t link, count;
-      if (t.children[node] >= 0) { link = (t.children[node] - 1) / 8; count = -1; }
-      else leaf_records(node, link, count);
+      if (t.children[node] >= 0) {
+        link = (t.children[node] - 1) / 8;
+        count = -1;
+        const int rank = __builtin_popcount(interior);
+        if (rank == 0) first_child = link;
+        else if (link != first_child + rank) return -1;   // not breadth-first
+        interior |= 1u << k;
+      } else {
+        leaf_records(node, link, count);
+      }
       memcpy(&G[kGroupLink + k], &link, 4);
       memcpy(&G[kGroupCount + k], &count, 4);
     }
-    memcpy(&G[kGroupMask], &nonempty, 4);
+    const uint32_t head = nonempty | (interior << 8);
+    memcpy(&G[kGroupMask], &head, 4);
+    memcpy(&G[kGroupBase], &first_child, 4);
   }
   out.nrec = (int)(out.recs.size() / kRecWords);
   return 0;

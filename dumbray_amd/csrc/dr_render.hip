@@ -20,6 +20,8 @@ namespace dr {
 constexpr int kRB = 256;               // lanes per workgroup
 constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per round
 constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
+constexpr int kLeafMin = 32;           // pending leaves that trigger the leaf tests
+constexpr int kShadeMin = 32;          // finished casts that trigger the path updates
 
 // Per-lane path state.
 struct Path {
@@ -106,7 +108,7 @@ __device__ __forceinline__ void reflect_factor(const dr_render_params &P, const 
 // ------------------------------------------------------------------ forward
 // LDS used by one block render.
 struct BlockLds {
-  uint32_t *stk;        // this lane's first stack word ([level][lane])
+  uint32_t *stk;        // this lane's first stack word ([word][lane])
   float *samp_col;      // kBlockPixels * kRoundSamples * 3 sample colours
   float *acc;           // kBlockPixels * 3 pixel sums
   int *next_job;
@@ -133,6 +135,8 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
     if (threadIdx.x == 0) *lds.next_job = 0;
     __syncthreads();
     Path st;
+    Walk w;
+    w.state = kWalkDone;
     bool exhausted = false;
     auto finish = [&]() {
       for (int c = 0; c < 3; c++) samp_col[3 * st.job + c] = st.L[c];
@@ -164,6 +168,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
             st.job = slot;
             const int lp = slot / ns, s = s_base + (slot - lp * ns);
             if (!start_path(P, block, lp, s, samp0, st)) { for (int c = 0; c < 3; c++) st.L[c] = 0.0f; finish(); }
+            else walk_begin(T, w, st.o, st.d, st.restrict_tri, lds.stk, kRB, nullptr);
           } else {
             exhausted = true;
           }
@@ -173,20 +178,30 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
         if (__ballot(!exhausted) == 0) break;
         continue;
       }
-      if (st.job >= 0) {
-        Hit hh;
-        const bool hit = tree_cast(T, st.o, st.d, clip, st.restrict_tri, lds.stk, kRB, hh);
-        if (st.phase == 0) {
-          if (!hit) finish();
-          else {
-            setup_vertex(P, shade, hh, st);
-            if (!(st.kind == DR_LAMBERT && next_photon(P, st))) advance();
+      // ---- one traversal step of every ray in flight
+      if (st.job >= 0 && w.state == kWalkTrace) walk_step(T, w, lds.stk, kRB, nullptr);
+      // ---- leaf tests, batched
+      const uint64_t pm = __ballot(st.job >= 0 && w.state == kWalkLeaf);
+      if (pm && (__popcll(pm) >= kLeafMin || __ballot(st.job >= 0 && w.state == kWalkTrace) == 0)) {
+        if (st.job >= 0 && w.state == kWalkLeaf) walk_leaf(T, w, clip, nullptr);
+      }
+      // ---- path updates of the finished casts, batched
+      const uint64_t dm = __ballot(st.job >= 0 && w.state == kWalkDone);
+      if (dm && (__popcll(dm) >= kShadeMin || __ballot(st.job >= 0 && w.state != kWalkDone) == 0)) {
+        if (st.job >= 0 && w.state == kWalkDone) {
+          if (st.phase == 0) {
+            if (!w.hit) finish();
+            else {
+              setup_vertex(P, shade, w.h, st);
+              if (!(st.kind == DR_LAMBERT && next_photon(P, st))) advance();
+            }
+          } else {
+            const V3 e = w.h.p - st.p;
+            const bool lit = w.hit && (w.h.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
+            if (lit) for (int c = 0; c < 3; c++) st.L[c] += st.contrib[c];
+            if (!next_photon(P, st)) advance();
           }
-        } else {
-          const V3 e = hh.p - st.p;
-          const bool lit = hit && (hh.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
-          if (lit) for (int c = 0; c < 3; c++) st.L[c] += st.contrib[c];
-          if (!next_photon(P, st)) advance();
+          if (st.job >= 0) walk_begin(T, w, st.o, st.d, st.restrict_tri, lds.stk, kRB, nullptr);
         }
       }
     }
@@ -221,7 +236,7 @@ __global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restr
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];
   __shared__ int next_job;
-  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kRB), acc, &next_job};
+  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job};
   render_block(T, shade, P, frame, start_block + blockIdx.x, L, 1);
 }
 
@@ -239,7 +254,7 @@ __global__ __launch_bounds__(kRB) void k_backward(DevTree T, const float4 *__res
   extern __shared__ uint32_t lds[];
   __shared__ int next_job;
   uint32_t *stk = lds + threadIdx.x;
-  float *vtx = (float *)(lds + levels * kRB);                     // maxv * kVtxWords * kRB
+  float *vtx = (float *)(lds + levels * kStackWordsPerLevel * kRB);   // maxv * kVtxWords * kRB
   float *gacc = vtx + (size_t)maxv * kVtxWords * kRB;              // nmat*3 albedo, then nlights*3 emission
   const int ngrad = 3 * (P.nmat + P.nlights);
   const int block = blockIdx.x;
@@ -375,7 +390,7 @@ __global__ __launch_bounds__(kRB) void k_batch(const DevRequest *__restrict__ re
   __shared__ float acc[kBlockPixels * 3];
   __shared__ int next_job;
   __shared__ int item;
-  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kRB), acc, &next_job};
+  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job};
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(counter, 1);
     __syncthreads();
@@ -413,7 +428,7 @@ extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *fr
     return dr::fail(DR_ERR_ARG, "dr_render_blocks: bad frame or block range");
   if (start_block == end_block) return DR_OK;
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
-  const size_t lds = ((size_t)levels * dr::kRB + (size_t)dr::kBlockPixels * dr::kRoundSamples * 3) * 4;
+  const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + (size_t)dr::kBlockPixels * dr::kRoundSamples * 3) * 4;
   hipLaunchKernelGGL(dr::k_render, dim3(end_block - start_block), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree,
                      (const float4 *)s->shade.p, *p, frame, start_block, levels);
   DR_HIP_CHECK(hipGetLastError());
@@ -427,7 +442,7 @@ extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *d
   if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return dr::fail(DR_ERR_ARG, "dr_backward: null buffers");
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const int maxv = p->max_bounces + 1;
-  const size_t lds = ((size_t)levels * dr::kRB + (size_t)maxv * dr::kVtxWords * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
+  const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + (size_t)maxv * dr::kVtxWords * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
   if (lds > 160 * 1024) return dr::fail(DR_ERR_ARG, "dr_backward: path too long or too many materials for LDS");
   if (lds > 64 * 1024) DR_HIP_CHECK(hipFuncSetAttribute((const void *)dr::k_backward,
                                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -451,7 +466,7 @@ struct dr_batch_s {
 namespace dr {
 
 static size_t batch_lds(int levels) {
-  return ((size_t)levels * kRB + (size_t)kBlockPixels * kRoundSamples * 3) * 4;
+  return ((size_t)levels * kStackWordsPerLevel * kRB + (size_t)kBlockPixels * kRoundSamples * 3) * 4;
 }
 
 // Queue of (request, first block, count), heaviest per-block cost first.

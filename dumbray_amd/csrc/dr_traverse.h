@@ -66,15 +66,13 @@ __device__ __forceinline__ bool inside(V3 p, V3 a, V3 b, V3 c, float thr0, float
 
 // castInLeaf: closest accepted triangle among `count` records; accepted iff its hit
 // point lies inside the leaf box.  exclude = validateNotSameAsObject's object (or -1).
-// Records are read DR_LEAF_CHUNK at a time so their loads are in flight together;
+// Records are read kLeafChunk at a time so their loads are in flight together;
 // the candidates of a chunk are then resolved in record order, which keeps the
 // reference's strict '<' first-wins rule.
-#ifndef DR_LEAF_CHUNK
-#define DR_LEAF_CHUNK 4
-#endif
+constexpr int kLeafChunk = 4;
 __device__ __forceinline__ bool leaf_cast(const float4 *__restrict__ R, int start, int count, const Box &lb, V3 o,
                                           V3 d, bool clip, int exclude, Hit &h, CastCounters *cc = nullptr) {
-  constexpr int C = DR_LEAF_CHUNK;
+  constexpr int C = kLeafChunk;
   if (cc) { cc->leaves++; cc->plane += count; }
   float best = FLT_MAX;
   V3 bp = v3(0.f, 0.f, 0.f);
@@ -125,10 +123,17 @@ __device__ __forceinline__ Box child_box(const float4 *__restrict__ G, int g, in
   return b;
 }
 
-// 8 child tests of group g -> mask in visiting order (bit i = child canon ^ i):
-// the child has a non-empty leaf below it, its box widened by the prune margin meets
-// the forward octant of the ray, and the reference slab test on its box passes.
-__device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int g, const TRay &r, float M) {
+// Traversal stack.  A level holds two words, [2*level][lane] and [2*level+1][lane]:
+//   lo = group << 8 | mask of children still to visit (bit i = child canon ^ i)
+//   hi = first interior child's group << 8 | interior mask (bit k = child k)
+// so popping an interior child needs no memory read.
+constexpr int kStackWordsPerLevel = 2;
+
+// 8 child tests of group g -> stack entry.  A child is kept if it has a non-empty
+// leaf below it, its box widened by the prune margin meets the forward octant of
+// the ray, and the reference slab test on its box passes.
+__device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g, const TRay &r, float M,
+                                            uint32_t &lo, uint32_t &hi) {
   const float4 *q = G + kGroupF4 * (size_t)g;
   uint32_t m = 0;
 #pragma unroll
@@ -147,12 +152,14 @@ __device__ __forceinline__ uint32_t group_mask(const float4 *__restrict__ G, int
       m |= hit ? (1u << (4 * h + j)) : 0u;
     }
   }
-  m &= ((const uint32_t *)q)[kGroupMask];
+  const uint2 head = *(const uint2 *)(q + kGroupMask / 4);
+  m &= head.x & 0xffu;
   const int c = r.canon;
   if (c & 1) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
   if (c & 2) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
   if (c & 4) m = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
-  return m;
+  lo = ((uint32_t)g << 8) | m;
+  hi = (head.y << 8) | ((head.x >> 8) & 0xffu);
 }
 
 // Pops the next child to visit.  Returns 1 with (g, k, link, cnt) set when a
@@ -163,22 +170,29 @@ __device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, float M, 
                                          CastCounters *cc = nullptr) {
   const int *Gi = (const int *)G;
   while (sp >= 0) {
-    const uint32_t f = stk[sp * stride];
+    uint32_t *e = stk + 2 * sp * stride;
+    const uint32_t f = e[0];
     const uint32_t m = f & 0xffu;
     if (m == 0) { --sp; continue; }
     const int i = __builtin_ctz(m);
-    stk[sp * stride] = f & ~(1u << i);
-    const int g = (int)(f >> 8);
+    e[0] = f & ~(1u << i);
     const int k = r.canon ^ i;
-    const size_t base = kGroupWords * (size_t)g;
-    const int link = Gi[base + kGroupLink + k];
-    const int cnt = Gi[base + kGroupCount + k];
-    if (cnt < 0) {
+    const uint32_t hi = e[stride];
+    const uint32_t im = hi & 0xffu;
+    if ((im >> k) & 1u) {
+      const int child = (int)(hi >> 8) + __builtin_popcount(im & ((1u << k) - 1u));
       ++sp;
-      stk[sp * stride] = ((uint32_t)link << 8) | group_mask(G, link, r, M);
+      uint32_t lo2, hi2;
+      group_entry(G, child, r, M, lo2, hi2);
+      stk[2 * sp * stride] = lo2;
+      stk[(2 * sp + 1) * stride] = hi2;
       if (cc) cc->expand++;
     } else {
-      g_out = g; k_out = k; link_out = link; cnt_out = cnt;
+      const int g = (int)(f >> 8);
+      const size_t base = kGroupWords * (size_t)g;
+      g_out = g; k_out = k;
+      link_out = Gi[base + kGroupLink + k];
+      cnt_out = Gi[base + kGroupCount + k];
       return 1;
     }
   }
@@ -186,23 +200,98 @@ __device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, float M, 
 }
 
 // Initial traversal state.  Returns 0 = miss, 1 = stack ready, 2 = root is a leaf.
-__device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint32_t *stk, int &sp,
+__device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint32_t *stk, int stride, int &sp,
                                           CastCounters *cc = nullptr) {
   if (T.root_is_leaf) return 2;
   if (!slab_hit(T.rsx, T.rsy, T.rsz, T.rex, T.rey, T.rez, r.o, r.inv)) return 0;
   sp = 0;
-  stk[0] = group_mask(T.groups, 0, r, T.margin);     // group 0 = the root's children
+  uint32_t lo, hi;
+  group_entry(T.groups, 0, r, T.margin, lo, hi);     // group 0 = the root's children
+  stk[0] = lo;
+  stk[stride] = hi;
   if (cc) cc->expand++;
   return 1;
 }
 
+// One lane's closest-hit query, advanced in bounded steps so that a kernel can
+// interleave many of them (and batch their leaf tests) across a wave.
+enum : int { kWalkTrace = 1, kWalkLeaf = 2, kWalkDone = 3 };
+struct Walk {
+  TRay ray;
+  int ex;              // validateNotSameAsObject's object, or -1
+  int sp, state;
+  int lg, lk, llink, lcnt;   // pending leaf: child lk of group lg (lg < 0: the root)
+  bool hit;            // result once state == kWalkDone
+  Hit h;
+};
+
+__device__ __forceinline__ void walk_begin(const DevTree &T, Walk &w, V3 o, V3 d, int exclude, uint32_t *stk,
+                                           int stride, CastCounters *cc) {
+  w.ray = make_tray(o, d);
+  w.ex = exclude;
+  w.hit = false;
+  const int st = tree_begin(T, w.ray, stk, stride, w.sp, cc);
+  if (st == 0) {
+    w.state = kWalkDone;
+  } else if (st == 2) {
+    w.state = kWalkLeaf;
+    w.lg = -1; w.llink = T.root_start; w.lcnt = T.root_count;
+  } else {
+    w.state = kWalkTrace;
+  }
+}
+
+// Pops one child: expands an interior child, or holds a leaf child as pending;
+// an exhausted stack ends the query with a miss.
+__device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *stk, int stride, CastCounters *cc) {
+  uint32_t f = 0;
+  while (w.sp >= 0) {
+    f = stk[2 * w.sp * stride];
+    if (f & 0xffu) break;
+    --w.sp;
+  }
+  if (w.sp < 0) { w.state = kWalkDone; return; }
+  const int i = __builtin_ctz(f & 0xffu);
+  stk[2 * w.sp * stride] = f & ~(1u << i);
+  const int k = w.ray.canon ^ i;
+  const uint32_t hi = stk[(2 * w.sp + 1) * stride];
+  const uint32_t im = hi & 0xffu;
+  if ((im >> k) & 1u) {
+    const int child = (int)(hi >> 8) + __builtin_popcount(im & ((1u << k) - 1u));
+    uint32_t lo2, hi2;
+    group_entry(T.groups, child, w.ray, T.margin, lo2, hi2);
+    ++w.sp;
+    stk[2 * w.sp * stride] = lo2;
+    stk[(2 * w.sp + 1) * stride] = hi2;
+    if (cc) cc->expand++;
+  } else {
+    const int *Gi = (const int *)T.groups;
+    const size_t base = kGroupWords * (size_t)(f >> 8);
+    w.lg = (int)(f >> 8);
+    w.lk = k;
+    w.llink = Gi[base + kGroupLink + k];
+    w.lcnt = Gi[base + kGroupCount + k];
+    w.state = kWalkLeaf;
+  }
+}
+
+// castInLeaf of the pending leaf; an accepted hit (or the root leaf) ends the query.
+__device__ __forceinline__ void walk_leaf(const DevTree &T, Walk &w, bool clip, CastCounters *cc) {
+  Box lb;
+  if (w.lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
+  else lb = child_box(T.groups, w.lg, w.lk);
+  w.hit = leaf_cast(T.recs, w.llink, w.lcnt, lb, w.ray.o, w.ray.d, clip, w.ex, w.h, cc);
+  w.state = (w.hit || w.lg < 0) ? kWalkDone : kWalkTrace;
+}
+
 // Octree::cast for one ray.  stk points at this lane's first stack word;
-// consecutive levels are `stride` words apart (LDS, [level][lane]).
+// consecutive stack words are `stride` apart (LDS, [word][lane]), and the stack
+// has kStackWordsPerLevel words per tree level.
 __device__ __forceinline__ bool tree_cast(const DevTree &T, V3 o, V3 d, bool clip, int exclude, uint32_t *stk,
                                           int stride, Hit &h) {
   const TRay r = make_tray(o, d);
   int sp = -1;
-  const int st = tree_begin(T, r, stk, sp);
+  const int st = tree_begin(T, r, stk, stride, sp);
   if (st == 0) return false;
   if (st == 2) {
     Box rb; rb.s = v3(T.rsx, T.rsy, T.rsz); rb.e = v3(T.rex, T.rey, T.rez);

@@ -59,3 +59,19 @@ def test_bad_knobs_raise():
     p.push(S.random_soup(5, seed=1))
     with pytest.raises(D.DumbRayError):
         p.build()
+
+
+@pytest.mark.parametrize("name,make,thresh,depth", CASES[:5], ids=[c[0] for c in CASES[:5]])
+def test_interior_children_own_consecutive_blocks(name, make, thresh, depth):
+    """The device group layout finds an interior child's group as the group of the
+    node's first interior child plus its rank: breadth-first export guarantees it."""
+    p = D.Octree(thresh, depth)
+    p.push(make())
+    p.build(nthreads=3)
+    _, children, _, _ = p.export()
+    for node in np.flatnonzero(children >= 0):
+        c = children[node]
+        sub = children[c:c + 8]
+        blocks = sub[sub >= 0]
+        if len(blocks):
+            assert np.array_equal(blocks, blocks[0] + 8 * np.arange(len(blocks))), node
