@@ -3,9 +3,10 @@
 // One 256-lane workgroup renders one 8x16 framebuffer block.  The block's work
 // is a list of path jobs (pixel, sample); idle lanes take the next jobs with a
 // wave64 ballot + mbcnt prefix and one LDS atomic per wave, so lanes whose
-// path ended early are regenerated instead of idling.  Every loop iteration
-// each busy lane casts exactly one ray (camera/bounce ray or light photon) with
-// the shared octree traversal and then advances its path.  Finished sample
+// path ended early are regenerated instead of idling.  Each lane's current ray
+// (camera/bounce ray or light photon) is a Walk advanced by one node pop per loop
+// iteration; leaf tests and the path updates of finished casts run once enough
+// lanes of the wave need them, which keeps the wave's lanes in step.  Finished sample
 // colours are staged in LDS and summed per pixel in the reference's subpixel
 // order before the block is written (set, or blended with 1/iteration).
 #include "dr_common.h"
@@ -22,6 +23,9 @@ constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per ro
 constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
 constexpr int kLeafMin = 32;           // pending leaves that trigger the leaf tests
 constexpr int kShadeMin = 32;          // finished casts that trigger the path updates
+// Waves per SIMD the render kernels are compiled for: the register cap this sets
+// (168 VGPRs) costs a few spills but beats 2 waves per SIMD clearly.
+constexpr int kRenderWaves = 3;
 
 // Per-lane path state.
 struct Path {
@@ -32,7 +36,7 @@ struct Path {
   int restrict_tri = -1;
   float beta[3], L[3];
   int h = -1, kind = 0, mid = 0;
-  V3 p, nu, du;
+  V3 p, nu;
   bool has_sample = false;
   V3 sdir;
   float sweight = 0.0f;
@@ -80,17 +84,17 @@ __device__ __forceinline__ bool next_photon(const dr_render_params &P, Path &st,
   return false;
 }
 
-// Vertex setup after a geometry hit.
+// Vertex setup after a geometry hit of the ray with direction `dir`.
 __device__ __forceinline__ void setup_vertex(const dr_render_params &P, const float4 *__restrict__ shade, const Hit &hh,
-                                             Path &st) {
+                                             V3 dir, Path &st) {
   st.h = hh.id;
   st.p = hh.p;
   const float4 sh = shade[hh.id];
   st.mid = __float_as_int(sh.w);
   st.kind = P.materials[st.mid].kind;
-  st.du = normalized(st.d);
-  st.nu = facing_normal(v3(sh.x, sh.y, sh.z), st.du);
-  st.has_sample = next_sample(P, st.kind, st.pix, st.samp, st.layer, st.nu, st.du, st.sdir, st.sweight);
+  const V3 du = normalized(dir);
+  st.nu = facing_normal(v3(sh.x, sh.y, sh.z), du);
+  st.has_sample = next_sample(P, st.kind, st.pix, st.samp, st.layer, st.nu, du, st.sdir, st.sweight);
   st.light = -1;
 }
 
@@ -189,18 +193,23 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
       const uint64_t dm = __ballot(st.job >= 0 && w.state == kWalkDone);
       if (dm && (__popcll(dm) >= kShadeMin || __ballot(st.job >= 0 && w.state != kWalkDone) == 0)) {
         if (st.job >= 0 && w.state == kWalkDone) {
+          // geometry hit: set up the vertex, then photons (Lambert) or the next bounce;
+          // photon: add its contribution if it reached the vertex, then the next photon.
+          bool photons = false;
           if (st.phase == 0) {
             if (!w.hit) finish();
             else {
-              setup_vertex(P, shade, w.h, st);
-              if (!(st.kind == DR_LAMBERT && next_photon(P, st))) advance();
+              setup_vertex(P, shade, w.h, w.ray.d, st);
+              photons = st.kind == DR_LAMBERT;
+              if (!photons) advance();
             }
           } else {
             const V3 e = w.h.p - st.p;
             const bool lit = w.hit && (w.h.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
             if (lit) for (int c = 0; c < 3; c++) st.L[c] += st.contrib[c];
-            if (!next_photon(P, st)) advance();
+            photons = true;
           }
+          if (photons && !next_photon(P, st)) advance();
           if (st.job >= 0) walk_begin(T, w, st.o, st.d, st.restrict_tri, lds.stk, kRB, nullptr);
         }
       }
@@ -231,7 +240,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kRB) void k_render(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
+__global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
                                                  float *__restrict__ frame, int start_block, int levels) {
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];
@@ -340,7 +349,7 @@ __global__ __launch_bounds__(kRB) void k_backward(DevTree T, const float4 *__res
       if (st.phase == 0) {
         if (!hit) finish();
         else {
-          setup_vertex(P, shade, hh, st);
+          setup_vertex(P, shade, hh, st.d, st);
           V_(K, 0) = __int_as_float(st.mid);
           for (int c = 0; c < 4; c++) V_(K, 4 + c) = 0.0f;
           K++;
@@ -384,7 +393,7 @@ __global__ void k_zero(int *counter) {
 }
 
 // Persistent workgroups drain the queue of (request, first block, block count) items.
-__global__ __launch_bounds__(kRB) void k_batch(const DevRequest *__restrict__ reqs, const int4 *__restrict__ items,
+__global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *__restrict__ reqs, const int4 *__restrict__ items,
                                                 int nitems, int *__restrict__ counter, int levels, int refill_min) {
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];

@@ -70,9 +70,9 @@ __device__ __forceinline__ bool inside(V3 p, V3 a, V3 b, V3 c, float thr0, float
 // the candidates of a chunk are then resolved in record order, which keeps the
 // reference's strict '<' first-wins rule.
 constexpr int kLeafChunk = 4;
+template <int C = kLeafChunk>
 __device__ __forceinline__ bool leaf_cast(const float4 *__restrict__ R, int start, int count, const Box &lb, V3 o,
                                           V3 d, bool clip, int exclude, Hit &h, CastCounters *cc = nullptr) {
-  constexpr int C = kLeafChunk;
   if (cc) { cc->leaves++; cc->plane += count; }
   float best = FLT_MAX;
   V3 bp = v3(0.f, 0.f, 0.f);
@@ -276,11 +276,12 @@ __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *s
 }
 
 // castInLeaf of the pending leaf; an accepted hit (or the root leaf) ends the query.
+template <int C = kLeafChunk>
 __device__ __forceinline__ void walk_leaf(const DevTree &T, Walk &w, bool clip, CastCounters *cc) {
   Box lb;
   if (w.lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
   else lb = child_box(T.groups, w.lg, w.lk);
-  w.hit = leaf_cast(T.recs, w.llink, w.lcnt, lb, w.ray.o, w.ray.d, clip, w.ex, w.h, cc);
+  w.hit = leaf_cast<C>(T.recs, w.llink, w.lcnt, lb, w.ray.o, w.ray.d, clip, w.ex, w.h, cc);
   w.state = (w.hit || w.lg < 0) ? kWalkDone : kWalkTrace;
 }
 
