@@ -250,23 +250,29 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, const f
 }
 
 // ------------------------------------------------------------------ adjoint
-// Vertex records of the lane's current path live in LDS ([vertex][word][lane]):
-// material id, reflection factor A (3), direct sum d (3) with L_k = A (d + s L_{k+1}),
-// and the sample weight s.  The path is replayed from its Philox key; after it ends
-// S_k = d_k + s_k L_{k+1} is formed backwards and dL/dalbedo += w dL/dI beta_k S_k.
-// dL/demission is added during the replay: w dL/dI beta_k A_k (cos_i/pi)(cos_l A/d^2).
+// Vertex records of each lane's current path ([vertex][word][lane] in the
+// workgroup's slice of a global scratch buffer): material id, reflection factor
+// A (3), direct sum d (3) with L_k = A (d + s L_{k+1}), and the sample weight s.
+// The path is replayed from its Philox key; after it ends S_k = d_k + s_k L_{k+1}
+// is formed backwards and dL/dalbedo += w dL/dI beta_k S_k.  dL/demission is
+// added during the replay: w dL/dI beta_k A_k (cos_i/pi)(cos_l A/d^2).
+// Rays are step-wise walks as in render_block.  The grid is persistent: workgroup
+// g handles framebuffer blocks g, g + gridDim.x, ... and keeps its gradient sums
+// in LDS until the end, then adds them with one FP64 atomic per parameter.
 constexpr int kVtxWords = 8;
 
-__global__ __launch_bounds__(kRB) void k_backward(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
-                                                   const float *__restrict__ dl, double *__restrict__ g_albedo,
-                                                   double *__restrict__ g_emission, int levels, int maxv) {
+__global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const float4 *__restrict__ shade,
+                                                                 dr_render_params P, const float *__restrict__ dl,
+                                                                 double *__restrict__ g_albedo,
+                                                                 double *__restrict__ g_emission,
+                                                                 float *__restrict__ vtx_all, int nblocks, int levels,
+                                                                 int maxv) {
   extern __shared__ uint32_t lds[];
   __shared__ int next_job;
   uint32_t *stk = lds + threadIdx.x;
-  float *vtx = (float *)(lds + levels * kStackWordsPerLevel * kRB);   // maxv * kVtxWords * kRB
-  float *gacc = vtx + (size_t)maxv * kVtxWords * kRB;              // nmat*3 albedo, then nlights*3 emission
+  float *gacc = (float *)(lds + levels * kStackWordsPerLevel * kRB);   // nmat*3 albedo, then nlights*3 emission
+  float *vtx = vtx_all + (size_t)blockIdx.x * maxv * kVtxWords * kRB;
   const int ngrad = 3 * (P.nmat + P.nlights);
-  const int block = blockIdx.x;
   const int spp = P.fsaa_x * P.fsaa_y;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
   const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
@@ -274,103 +280,119 @@ __global__ __launch_bounds__(kRB) void k_backward(DevTree T, const float4 *__res
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int t = threadIdx.x; t < ngrad; t += kRB) gacc[t] = 0.0f;
-  if (threadIdx.x == 0) next_job = 0;
-  __syncthreads();
   const int njobs = kBlockPixels * spp;
   auto V_ = [&](int k, int w) -> float & { return vtx[((size_t)k * kVtxWords + w) * kRB + threadIdx.x]; };
 
-  Path st;
-  int K = 0;                      // vertices recorded for the current path
-  float wdl[3] = {0, 0, 0};       // spw * dL/dI of the current pixel
-  bool exhausted = false;
-  auto finish = [&]() {
-    float Ln[3] = {0, 0, 0};
-    for (int k = K - 1; k >= 0; k--)
-      for (int c = 0; c < 3; c++) {
-        const float S = V_(k, 4 + c) + V_(k, 7) * Ln[c];
-        Ln[c] = V_(k, 1 + c) * S;
-        V_(k, 4 + c) = S;
-      }
-    float beta[3] = {1, 1, 1};
-    for (int k = 0; k < K; k++) {
-      const int m = __float_as_int(V_(k, 0));
-      const int kind = P.materials[m].kind;
-      for (int c = 0; c < 3; c++) {
-        if (kind != DR_METAL) atomicAdd(&gacc[3 * m + c], wdl[c] * beta[c] * V_(k, 4 + c));
-        beta[c] = beta[c] * (V_(k, 1 + c) * V_(k, 7));
-      }
-    }
-    st.job = -1;
-  };
-  auto advance = [&]() {
-    float A[3];
-    reflect_factor(P, st, A);
-    for (int c = 0; c < 3; c++) V_(K - 1, 1 + c) = A[c];
-    V_(K - 1, 7) = st.has_sample ? st.sweight : 0.0f;
-    if (!st.has_sample) { finish(); return; }
-    for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (A[c] * st.sweight);
-    st.o = st.p;
-    st.d = st.sdir;
-    st.restrict_tri = st.h;
-    st.layer++;
-    st.phase = 0;
-  };
-  for (;;) {
-    const bool want = (st.job < 0) && !exhausted;
-    const uint64_t wmask = __ballot(want);
-    if (wmask) {
-      const int leader = __ffsll((unsigned long long)wmask) - 1;
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&next_job, __popcll(wmask));
-      base = __shfl(base, leader);
-      if (want) {
-        const int slot = base + __popcll(wmask & below);
-        if (slot < njobs) {
-          st.job = slot;
-          const int lp = slot / spp, s = slot - lp * spp;
-          K = 0;
-          if (!start_path(P, block, lp, s, samp0, st)) st.job = -1;
-          else {
-            const float *g = dl + 4 * ((size_t)kBlockPixels * block + lp);
-            for (int c = 0; c < 3; c++) wdl[c] = spw * g[c];
-          }
-        } else {
-          exhausted = true;
+  for (int block = blockIdx.x; block < nblocks; block += gridDim.x) {
+    if (threadIdx.x == 0) next_job = 0;
+    __syncthreads();
+    Path st;
+    Walk w;
+    w.state = kWalkDone;
+    int K = 0;                      // vertices recorded for the current path
+    float wdl[3] = {0, 0, 0};       // spw * dL/dI of the current pixel
+    bool exhausted = false;
+    auto finish = [&]() {
+      float Ln[3] = {0, 0, 0};
+      for (int k = K - 1; k >= 0; k--)
+        for (int c = 0; c < 3; c++) {
+          const float S = V_(k, 4 + c) + V_(k, 7) * Ln[c];
+          Ln[c] = V_(k, 1 + c) * S;
+          V_(k, 4 + c) = S;
+        }
+      float beta[3] = {1, 1, 1};
+      for (int k = 0; k < K; k++) {
+        const int m = __float_as_int(V_(k, 0));
+        const int kind = P.materials[m].kind;
+        for (int c = 0; c < 3; c++) {
+          if (kind != DR_METAL) atomicAdd(&gacc[3 * m + c], wdl[c] * beta[c] * V_(k, 4 + c));
+          beta[c] = beta[c] * (V_(k, 1 + c) * V_(k, 7));
         }
       }
-    }
-    if (__ballot(st.job >= 0) == 0) {
-      if (__ballot(!exhausted) == 0) break;
-      continue;
-    }
-    if (st.job >= 0) {
-      Hit hh;
-      const bool hit = tree_cast(T, st.o, st.d, clip, st.restrict_tri, stk, kRB, hh);
-      if (st.phase == 0) {
-        if (!hit) finish();
-        else {
-          setup_vertex(P, shade, hh, st.d, st);
-          V_(K, 0) = __int_as_float(st.mid);
-          for (int c = 0; c < 4; c++) V_(K, 4 + c) = 0.0f;
-          K++;
-          if (!(st.kind == DR_LAMBERT && next_photon(P, st))) advance();
-        }
-      } else {
-        const V3 e = hh.p - st.p;
-        const bool lit = hit && (hh.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
-        if (lit) {
-          const dr_material M = P.materials[st.mid];
-          const float *E = P.lights[st.light].emission;
-          for (int c = 0; c < 3; c++) {
-            V_(K - 1, 4 + c) += st.gl_k * E[c];
-            atomicAdd(&gacc[3 * P.nmat + 3 * st.light + c], wdl[c] * st.beta[c] * M.albedo[c] * st.gl_k);
+      st.job = -1;
+    };
+    auto advance = [&]() {
+      float A[3];
+      reflect_factor(P, st, A);
+      for (int c = 0; c < 3; c++) V_(K - 1, 1 + c) = A[c];
+      V_(K - 1, 7) = st.has_sample ? st.sweight : 0.0f;
+      if (!st.has_sample) { finish(); return; }
+      for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (A[c] * st.sweight);
+      st.o = st.p;
+      st.d = st.sdir;
+      st.restrict_tri = st.h;
+      st.layer++;
+      st.phase = 0;
+    };
+    for (;;) {
+      const bool want = (st.job < 0) && !exhausted;
+      const uint64_t wmask = __ballot(want);
+      if (wmask) {
+        const int leader = __ffsll((unsigned long long)wmask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&next_job, __popcll(wmask));
+        base = __shfl(base, leader);
+        if (want) {
+          const int slot = base + __popcll(wmask & below);
+          if (slot < njobs) {
+            st.job = slot;
+            const int lp = slot / spp, s = slot - lp * spp;
+            K = 0;
+            if (!start_path(P, block, lp, s, samp0, st)) {
+              st.job = -1;
+            } else {
+              const float *g = dl + 4 * ((size_t)kBlockPixels * block + lp);
+              for (int c = 0; c < 3; c++) wdl[c] = spw * g[c];
+              walk_begin(T, w, st.o, st.d, st.restrict_tri, stk, kRB, nullptr);
+            }
+          } else {
+            exhausted = true;
           }
         }
-        if (!next_photon(P, st)) advance();
+      }
+      if (__ballot(st.job >= 0) == 0) {
+        if (__ballot(!exhausted) == 0) break;
+        continue;
+      }
+      if (st.job >= 0 && w.state == kWalkTrace) walk_step(T, w, stk, kRB, nullptr);
+      const uint64_t pm = __ballot(st.job >= 0 && w.state == kWalkLeaf);
+      if (pm && (__popcll(pm) >= kLeafMin || __ballot(st.job >= 0 && w.state == kWalkTrace) == 0)) {
+        if (st.job >= 0 && w.state == kWalkLeaf) walk_leaf(T, w, clip, nullptr);
+      }
+      const uint64_t dm = __ballot(st.job >= 0 && w.state == kWalkDone);
+      if (dm && (__popcll(dm) >= kShadeMin || __ballot(st.job >= 0 && w.state != kWalkDone) == 0)) {
+        if (st.job >= 0 && w.state == kWalkDone) {
+          bool photons = false;
+          if (st.phase == 0) {
+            if (!w.hit) finish();
+            else {
+              setup_vertex(P, shade, w.h, w.ray.d, st);
+              V_(K, 0) = __int_as_float(st.mid);
+              for (int c = 0; c < 4; c++) V_(K, 4 + c) = 0.0f;
+              K++;
+              photons = st.kind == DR_LAMBERT;
+              if (!photons) advance();
+            }
+          } else {
+            const V3 e = w.h.p - st.p;
+            const bool lit = w.hit && (w.h.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
+            if (lit) {
+              const dr_material M = P.materials[st.mid];
+              const float *E = P.lights[st.light].emission;
+              for (int c = 0; c < 3; c++) {
+                V_(K - 1, 4 + c) += st.gl_k * E[c];
+                atomicAdd(&gacc[3 * P.nmat + 3 * st.light + c], wdl[c] * st.beta[c] * M.albedo[c] * st.gl_k);
+              }
+            }
+            photons = true;
+          }
+          if (photons && !next_photon(P, st)) advance();
+          if (st.job >= 0) walk_begin(T, w, st.o, st.d, st.restrict_tri, stk, kRB, nullptr);
+        }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int t = threadIdx.x; t < ngrad; t += kRB) {
     const float v = gacc[t];
     if (v != 0.0f) {
@@ -451,13 +473,26 @@ extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *d
   if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return dr::fail(DR_ERR_ARG, "dr_backward: null buffers");
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const int maxv = p->max_bounces + 1;
-  const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + (size_t)maxv * dr::kVtxWords * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
-  if (lds > 160 * 1024) return dr::fail(DR_ERR_ARG, "dr_backward: path too long or too many materials for LDS");
+  const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
+  if (lds > 160 * 1024) return dr::fail(DR_ERR_ARG, "dr_backward: too many materials and lights for LDS");
   if (lds > 64 * 1024) DR_HIP_CHECK(hipFuncSetAttribute((const void *)dr::k_backward,
                                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(dr::k_backward, dim3(dr::block_count(p)), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree,
-                     (const float4 *)s->shade.p, *p, dl_dframe, g_albedo, g_emission, levels, maxv);
-  DR_HIP_CHECK(hipGetLastError());
+  const int nblocks = dr::block_count(p);
+  if (nblocks == 0) return DR_OK;
+  int per_cu = 0;
+  DR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dr::k_backward, dr::kRB, lds));
+  const int groups = std::max(1, std::min(nblocks, s->num_cu * std::max(per_cu, 1)));
+  // vertex records: one slice of maxv * kVtxWords * kRB floats per workgroup, stream-ordered
+  const size_t vbytes = (size_t)groups * maxv * dr::kVtxWords * dr::kRB * sizeof(float);
+  void *vtx = nullptr;
+  DR_HIP_CHECK(hipMallocAsync(&vtx, vbytes, (hipStream_t)stream));
+  hipLaunchKernelGGL(dr::k_backward, dim3(groups), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree,
+                     (const float4 *)s->shade.p, *p, dl_dframe, g_albedo, g_emission, (float *)vtx, nblocks, levels,
+                     maxv);
+  const hipError_t le = hipGetLastError();
+  const hipError_t fe = hipFreeAsync(vtx, (hipStream_t)stream);
+  DR_HIP_CHECK(le);
+  DR_HIP_CHECK(fe);
   return DR_OK;
 }
 
