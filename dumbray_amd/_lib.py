@@ -36,6 +36,7 @@ def _declare(L):
     optional = {
         "dr_cast_stats": (c_int, [vp, vp, c_int, c_uint, vp, vp, vp, vp, vp]),
         "dr_scene_set_materials": (c_int, [vp, vp, c_int]),
+        "dr_scene_set_normals": (c_int, [vp, vp, c_int]),
         "dr_render_blocks": (c_int, [vp, vp, vp, c_int, c_int, vp]),
         "dr_backward": (c_int, [vp, vp, vp, vp, vp, vp]),
         "dr_batch_create": (c_int, [vp, c_int, vp, vp]),

@@ -2,6 +2,7 @@
 #include <cstring>
 #include <new>
 #include "dr_common.h"
+#include "dr_plugins.h"
 
 namespace dr {
 static thread_local std::string g_err = "";
@@ -121,13 +122,36 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
   s->max_depth = flat.max_depth;
   s->ntris = (int)t->host.tris.size();
   s->shade_host.resize(s->ntris);
+  s->faces_host.resize((size_t)s->ntris * dr::kFaceWords);
   for (int i = 0; i < s->ntris; i++) {
     const dr::Tri &tr = t->host.tris[i];
     const dr::V3 n = dr::cross(tr.b - tr.a, tr.c - tr.a);
     s->shade_host[i] = make_float4(n.x, n.y, n.z, 0.0f);
+    float *F = &s->faces_host[(size_t)i * dr::kFaceWords];
+    const float v[9] = {tr.a.x, tr.a.y, tr.a.z, tr.b.x, tr.b.y, tr.b.z, tr.c.x, tr.c.y, tr.c.z};
+    memcpy(F, v, sizeof(v));
   }
+  rc = dr_scene_set_normals(s, nullptr, s->ntris);
+  if (rc != DR_OK) { dr_scene_destroy(s); return rc; }
   *out = s;
   return DR_OK;
+}
+
+int dr_scene_set_normals(dr_scene s, const float *normals, int ntris) {
+  if (!s || ntris != s->ntris) return dr::fail(DR_ERR_ARG, "dr_scene_set_normals: one normal triple per triangle required");
+  for (int i = 0; i < ntris; i++) {
+    float *F = &s->faces_host[(size_t)i * dr::kFaceWords];
+    if (normals) {
+      memcpy(F + 9, normals + 9 * (size_t)i, 9 * sizeof(float));
+    } else {
+      // flat: the unit Triangle::normal (c - a) x (b - a) at every vertex
+      const dr::V3 a = dr::v3(F[0], F[1], F[2]), b = dr::v3(F[3], F[4], F[5]), c = dr::v3(F[6], F[7], F[8]);
+      const dr::V3 n = dr::normalized(dr::cross(c - a, b - a));
+      for (int k = 0; k < 3; k++) { F[9 + 3 * k] = n.x; F[10 + 3 * k] = n.y; F[11 + 3 * k] = n.z; }
+    }
+  }
+  dr::dev_free(s->faces);
+  return dr::dev_upload(s->faces, s->faces_host.data(), s->faces_host.size() * sizeof(float));
 }
 
 int dr_scene_set_materials(dr_scene s, const int32_t *mat_id, int ntris) {
@@ -152,6 +176,7 @@ void dr_scene_destroy(dr_scene s) {
   dr::dev_free(s->recs);
   dr::dev_free(s->shade);
   dr::dev_free(s->work);
+  dr::dev_free(s->faces);
   delete s;
 }
 

@@ -10,6 +10,7 @@
 // colours are staged in LDS and summed per pixel in the reference's subpixel
 // order before the block is written (set, or blended with 1/iteration).
 #include "dr_common.h"
+#include "dr_plugins.h"
 #include "dr_shade.h"
 
 #include <algorithm>
@@ -109,6 +110,207 @@ __device__ __forceinline__ void reflect_factor(const dr_render_params &P, const 
   }
 }
 
+// Shading tables of one scene.
+struct Tables {
+  const float4 *shade;   // per triangle: (n, material id bits)
+  const float *faces;    // per triangle: kFaceWords floats (vertices, vertex normals)
+};
+
+// Path updates of the benchmark plugins (DR_PLUGINS_BENCHMARK).
+struct BenchPath {
+  Path st;
+  __device__ __forceinline__ bool start(const dr_render_params &P, int block, int lp, int s, uint32_t samp0) {
+    return start_path(P, block, lp, s, samp0, st);
+  }
+  __device__ __forceinline__ V3 ray_o() const { return st.o; }
+  __device__ __forceinline__ V3 ray_d() const { return st.d; }
+  __device__ __forceinline__ int ray_excl() const { return st.restrict_tri; }
+  __device__ __forceinline__ const float *color() const { return st.L; }
+  // Next bounce after the last photon of a vertex; false when the path ends.
+  __device__ __forceinline__ bool advance(const dr_render_params &P) {
+    if (!st.has_sample) return false;
+    float A[3];
+    reflect_factor(P, st, A);
+    for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (A[c] * st.sweight);
+    st.o = st.p;
+    st.d = st.sdir;
+    st.restrict_tri = st.h;
+    st.layer++;
+    st.phase = 0;
+    return true;
+  }
+  // After the lane's cast ended.  A geometry hit sets up the vertex, then photons
+  // (Lambert) or the next bounce; a photon adds its contribution if it reached the
+  // vertex, then the next photon.  Returns true when the path is done.
+  __device__ __forceinline__ bool shade(const dr_render_params &P, const Tables &tb, const Walk &w) {
+    bool photons = false;
+    if (st.phase == 0) {
+      if (!w.hit) return true;
+      setup_vertex(P, tb.shade, w.h, w.ray.d, st);
+      photons = st.kind == DR_LAMBERT;
+    } else {
+      const V3 e = w.h.p - st.p;
+      const bool lit = w.hit && (w.h.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
+      if (lit) for (int c = 0; c < 3; c++) st.L[c] += st.contrib[c];
+      photons = true;
+    }
+    if (photons && next_photon(P, st)) return false;
+    return !advance(P);
+  }
+};
+
+// Path updates of the reference estimator (DR_PLUGINS_REFERENCE), DumbRenderer.cu:256-443:
+// every layer casts one photon per light; a photon that first reaches another surface
+// continues through it with that surface's getReflectedColor (observer = the photon
+// direction) as its colour when that colour is not black (:396-413).  Colours are
+// carried forward: L += beta * (getReflectedColor of each direct photon), and the
+// child layer's throughput is beta * getReflectedColor(indirect photon) * sampleWeight.
+struct RefPath {
+  uint32_t pix, samp;
+  int layer, phase, light, excl, trans;
+  V3 o, d;                          // the next ray to cast
+  float beta[3], L[3];
+  int h, mid, stype;                // current vertex
+  V3 din, nu;
+  RefHit H;
+  bool has;
+  V3 sdir;
+  float bnext[3];                   // throughput of the child layer
+  V3 pd;                            // current photon: direction, colour, unit direction?
+  float pc[3];
+  bool unit;
+
+  __device__ __forceinline__ bool start(const dr_render_params &P, int block, int lp, int s, uint32_t samp0) {
+    int x, y;
+    if (!block_pixel(P.width, P.height, block, lp, x, y)) return false;
+    pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
+    samp = samp0 + (uint32_t)s;
+    const int i = s / P.fsaa_y, j = s - (s / P.fsaa_y) * P.fsaa_y;
+    float u[4];
+    rng4(P, pix, samp, 0, 0, u);
+    camera_ray(P, x, y, i, j, u[0], u[1], o, d);
+    layer = 0;
+    phase = 0;
+    excl = -1;
+    for (int c = 0; c < 3; c++) { beta[c] = 1.0f; L[c] = 0.0f; }
+    return true;
+  }
+  __device__ __forceinline__ V3 ray_o() const { return o; }
+  __device__ __forceinline__ V3 ray_d() const { return d; }
+  __device__ __forceinline__ int ray_excl() const { return excl; }
+  __device__ __forceinline__ const float *color() const { return L; }
+
+  // Vertex after a geometry hit: the indirect sample and the child throughput.
+  __device__ __forceinline__ void setup(const dr_render_params &P, const Tables &tb, const Walk &w) {
+    h = w.h.id;
+    din = w.ray.d;
+    const float4 sh = tb.shade[h];
+    mid = __float_as_int(sh.w);
+    const dr_material M = P.materials[mid];
+    const V3 du = normalized(din);
+    nu = facing_normal(v3(sh.x, sh.y, sh.z), du);
+    H.p = w.h.p;
+    if (is_reference_kind(M.kind)) H = ref_hit(tb.faces, h, w.h.p);
+    has = false;
+    stype = 0;
+    float sw = 1.0f;
+    if (layer < P.max_bounces) {
+      float u[4];
+      rng4(P, pix, samp, (uint32_t)layer, 1, u);
+      bool alive = true;
+      if (layer >= P.rr_start) {
+        if (u[2] >= P.rr_survive) alive = false;
+        else sw = 1.0f / P.rr_survive;
+      }
+      if (alive) {
+        if (M.kind == DR_LAMBERT) { sdir = cosine_dir(nu, u[0], u[1]); has = true; }
+        else if (M.kind == DR_MIRROR || M.kind == DR_METAL) { sdir = reflect_unit(du, nu); has = true; }
+        else has = sample_ref(P, M, H, din, pix, samp, layer, u, sdir, stype);
+      }
+    }
+    if (has) {
+      float F[3];
+      if (is_reference_kind(M.kind)) {
+        const float white[3] = {1.0f, 1.0f, 1.0f};
+        reflect_ref(M, H, neg(sdir), white, neg(din), false, stype, F);
+      } else if (M.kind == DR_METAL) {
+        const float f5 = schlick5(nu, sdir);
+        for (int c = 0; c < 3; c++) F[c] = M.albedo[c] + (1.0f - M.albedo[c]) * f5;
+      } else {
+        for (int c = 0; c < 3; c++) F[c] = M.albedo[c];
+      }
+      for (int c = 0; c < 3; c++) bnext[c] = beta[c] * (F[c] * sw);
+    }
+    light = (M.kind == DR_MIRROR || M.kind == DR_METAL) ? P.nlights : -1;   // delta BSDFs take no direct light
+  }
+  // Direct light of the current photon at the vertex.
+  __device__ __forceinline__ void add_direct(const dr_render_params &P) {
+    const dr_material M = P.materials[mid];
+    if (M.kind == DR_LAMBERT) {
+      const V3 wn = unit ? pd : normalized(pd);
+      const float cos_i = -dot(nu, wn);
+      if (cos_i > 0.0f) {
+        const float k = cos_i * kInvPi;
+        for (int c = 0; c < 3; c++) L[c] += beta[c] * ((M.albedo[c] * k) * pc[c]);
+      }
+    } else {
+      float f[3];
+      reflect_ref(M, H, pd, pc, neg(din), true, 0, f);
+      for (int c = 0; c < 3; c++) L[c] += beta[c] * f[c];
+    }
+  }
+  // After the lane's cast ended; true when the path is done.
+  __device__ __forceinline__ bool shade(const dr_render_params &P, const Tables &tb, const Walk &w) {
+    if (phase == 0) {
+      if (!w.hit) return true;                        // the layer ray missed (:418-421)
+      setup(P, tb, w);
+    } else if (w.hit) {
+      const V3 e = w.h.p - H.p;
+      if (w.h.id == h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps)) {
+        add_direct(P);
+      } else {
+        const int hb = w.h.id;
+        const dr_material Mb = P.materials[__float_as_int(tb.shade[hb].w)];
+        if (is_reference_kind(Mb.kind) && trans < kMaxTransmit) {
+          const RefHit B = ref_hit(tb.faces, hb, w.h.p);
+          float tc[3];
+          reflect_ref(Mb, B, pd, pc, pd, true, 0, tc);
+          if (((tc[0] + tc[1]) + tc[2]) > kVecEps) {
+            for (int c = 0; c < 3; c++) pc[c] = tc[c];
+            o = w.h.p + normalized(pd) * (64.0f * kVecEps);
+            d = pd;
+            excl = hb;
+            trans++;
+            return false;
+          }
+        }
+      }
+    }
+    for (int l = light + 1; l < P.nlights; l++) {
+      V3 q, wv;
+      bool u1;
+      if (!photon_ref(P, P.lights[l], l, pix, samp, layer, H.p, q, wv, pc, u1)) continue;
+      light = l;
+      pd = wv;
+      unit = u1;
+      o = q;
+      d = wv;
+      excl = -1;
+      trans = 0;
+      phase = 1;
+      return false;
+    }
+    if (!has) return true;
+    for (int c = 0; c < 3; c++) beta[c] = bnext[c];
+    o = H.p;
+    d = sdir;
+    excl = h;
+    layer++;
+    phase = 0;
+    return false;
+  }
+};
+
 // ------------------------------------------------------------------ forward
 // LDS used by one block render.
 struct BlockLds {
@@ -121,9 +323,10 @@ struct BlockLds {
 // Renders framebuffer block `block` of one request (all lanes of the workgroup call it).
 // refill_min: an idle lane takes a new job once at least this many lanes of its wave
 // are idle (or the whole wave is).
-__device__ __forceinline__ void render_block(const DevTree &T, const float4 *__restrict__ shade,
-                                             const dr_render_params &P, float *__restrict__ frame, int block,
-                                             const BlockLds &lds, int refill_min) {
+template <class PathT>
+__device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb, const dr_render_params &P,
+                                             float *__restrict__ frame, int block, const BlockLds &lds,
+                                             int refill_min) {
   const int spp = P.fsaa_x * P.fsaa_y;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
   const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
@@ -138,29 +341,19 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
     const int njobs = kBlockPixels * ns;       // job j: local pixel j / ns, sample s_base + j % ns
     if (threadIdx.x == 0) *lds.next_job = 0;
     __syncthreads();
-    Path st;
+    PathT ps;
+    int job = -1;
     Walk w;
     w.state = kWalkDone;
     bool exhausted = false;
-    auto finish = [&]() {
-      for (int c = 0; c < 3; c++) samp_col[3 * st.job + c] = st.L[c];
-      st.job = -1;
-    };
-    auto advance = [&]() {           // after the last photon of a vertex
-      if (!st.has_sample) { finish(); return; }
-      float A[3];
-      reflect_factor(P, st, A);
-      for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (A[c] * st.sweight);
-      st.o = st.p;
-      st.d = st.sdir;
-      st.restrict_tri = st.h;
-      st.layer++;
-      st.phase = 0;
+    auto finish = [&](const float *L) {
+      for (int c = 0; c < 3; c++) samp_col[3 * job + c] = L[c];
+      job = -1;
     };
     for (;;) {
-      const bool want = (st.job < 0) && !exhausted;
+      const bool want = (job < 0) && !exhausted;
       const uint64_t wmask = __ballot(want);
-      const uint64_t busy = __ballot(st.job >= 0);
+      const uint64_t busy = __ballot(job >= 0);
       if (wmask && (__popcll(wmask) >= refill_min || busy == 0)) {
         const int leader = __ffsll((unsigned long long)wmask) - 1;
         int base = 0;
@@ -169,48 +362,36 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
         if (want) {
           const int slot = base + __popcll(wmask & below);
           if (slot < njobs) {
-            st.job = slot;
+            job = slot;
             const int lp = slot / ns, s = s_base + (slot - lp * ns);
-            if (!start_path(P, block, lp, s, samp0, st)) { for (int c = 0; c < 3; c++) st.L[c] = 0.0f; finish(); }
-            else walk_begin(T, w, st.o, st.d, st.restrict_tri, lds.stk, kRB, nullptr);
+            if (!ps.start(P, block, lp, s, samp0)) {
+              const float zero[3] = {0.0f, 0.0f, 0.0f};
+              finish(zero);
+            } else {
+              walk_begin(T, w, ps.ray_o(), ps.ray_d(), ps.ray_excl(), lds.stk, kRB, nullptr);
+            }
           } else {
             exhausted = true;
           }
         }
       }
-      if (__ballot(st.job >= 0) == 0) {
+      if (__ballot(job >= 0) == 0) {
         if (__ballot(!exhausted) == 0) break;
         continue;
       }
       // ---- one traversal step of every ray in flight
-      if (st.job >= 0 && w.state == kWalkTrace) walk_step(T, w, lds.stk, kRB, nullptr);
+      if (job >= 0 && w.state == kWalkTrace) walk_step(T, w, lds.stk, kRB, nullptr);
       // ---- leaf tests, batched
-      const uint64_t pm = __ballot(st.job >= 0 && w.state == kWalkLeaf);
-      if (pm && (__popcll(pm) >= kLeafMin || __ballot(st.job >= 0 && w.state == kWalkTrace) == 0)) {
-        if (st.job >= 0 && w.state == kWalkLeaf) walk_leaf(T, w, clip, nullptr);
+      const uint64_t pm = __ballot(job >= 0 && w.state == kWalkLeaf);
+      if (pm && (__popcll(pm) >= kLeafMin || __ballot(job >= 0 && w.state == kWalkTrace) == 0)) {
+        if (job >= 0 && w.state == kWalkLeaf) walk_leaf(T, w, clip, nullptr);
       }
       // ---- path updates of the finished casts, batched
-      const uint64_t dm = __ballot(st.job >= 0 && w.state == kWalkDone);
-      if (dm && (__popcll(dm) >= kShadeMin || __ballot(st.job >= 0 && w.state != kWalkDone) == 0)) {
-        if (st.job >= 0 && w.state == kWalkDone) {
-          // geometry hit: set up the vertex, then photons (Lambert) or the next bounce;
-          // photon: add its contribution if it reached the vertex, then the next photon.
-          bool photons = false;
-          if (st.phase == 0) {
-            if (!w.hit) finish();
-            else {
-              setup_vertex(P, shade, w.h, w.ray.d, st);
-              photons = st.kind == DR_LAMBERT;
-              if (!photons) advance();
-            }
-          } else {
-            const V3 e = w.h.p - st.p;
-            const bool lit = w.hit && (w.h.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
-            if (lit) for (int c = 0; c < 3; c++) st.L[c] += st.contrib[c];
-            photons = true;
-          }
-          if (photons && !next_photon(P, st)) advance();
-          if (st.job >= 0) walk_begin(T, w, st.o, st.d, st.restrict_tri, lds.stk, kRB, nullptr);
+      const uint64_t dm = __ballot(job >= 0 && w.state == kWalkDone);
+      if (dm && (__popcll(dm) >= kShadeMin || __ballot(job >= 0 && w.state != kWalkDone) == 0)) {
+        if (job >= 0 && w.state == kWalkDone) {
+          if (ps.shade(P, tb, w)) finish(ps.color());
+          else walk_begin(T, w, ps.ray_o(), ps.ray_d(), ps.ray_excl(), lds.stk, kRB, nullptr);
         }
       }
     }
@@ -240,13 +421,15 @@ __device__ __forceinline__ void render_block(const DevTree &T, const float4 *__r
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, const float4 *__restrict__ shade, dr_render_params P,
-                                                 float *__restrict__ frame, int start_block, int levels) {
+template <class PathT>
+__global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables tb, dr_render_params P,
+                                                               float *__restrict__ frame, int start_block,
+                                                               int levels) {
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];
   __shared__ int next_job;
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job};
-  render_block(T, shade, P, frame, start_block + blockIdx.x, L, 1);
+  render_block<PathT>(T, tb, P, frame, start_block + blockIdx.x, L, 1);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -430,7 +613,8 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
     if (it >= nitems) break;
     const int4 w = items[it];
     const DevRequest &R = reqs[w.x];
-    for (int b = w.y; b < w.y + w.z; b++) render_block(R.T, R.shade, R.P, R.frame, b, L, refill_min);
+    const Tables tb = {R.shade, nullptr};
+    for (int b = w.y; b < w.y + w.z; b++) render_block<BenchPath>(R.T, tb, R.P, R.frame, b, L, refill_min);
   }
 }
 
@@ -444,6 +628,9 @@ static int check_params(dr_scene s, const dr_render_params *p) {
   if (!p->materials || p->nmat <= s->max_mat) return fail(DR_ERR_ARG, "render: material table smaller than the ids in use");
   if (!(p->rr_survive > 0.0f && p->rr_survive <= 1.0f)) return fail(DR_ERR_ARG, "render: rr_survive must be in (0, 1]");
   if (s->max_depth > kMaxTreeDepth) return fail(DR_ERR_ARG, "render: octree deeper than the traversal stack");
+  if (p->lens != DR_STOCHASTIC_LENS && p->lens != DR_PERSPECTIVE_LENS) return fail(DR_ERR_ARG, "render: unknown lens");
+  if (p->plugins != DR_PLUGINS_BENCHMARK && p->plugins != DR_PLUGINS_REFERENCE)
+    return fail(DR_ERR_ARG, "render: unknown plugin set");
   return DR_OK;
 }
 
@@ -460,8 +647,13 @@ extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *fr
   if (start_block == end_block) return DR_OK;
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + (size_t)dr::kBlockPixels * dr::kRoundSamples * 3) * 4;
-  hipLaunchKernelGGL(dr::k_render, dim3(end_block - start_block), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree,
-                     (const float4 *)s->shade.p, *p, frame, start_block, levels);
+  const dr::Tables tb = {(const float4 *)s->shade.p, (const float *)s->faces.p};
+  if (p->plugins == DR_PLUGINS_REFERENCE)
+    hipLaunchKernelGGL(dr::k_render<dr::RefPath>, dim3(end_block - start_block), dim3(dr::kRB), lds,
+                       (hipStream_t)stream, s->tree, tb, *p, frame, start_block, levels);
+  else
+    hipLaunchKernelGGL(dr::k_render<dr::BenchPath>, dim3(end_block - start_block), dim3(dr::kRB), lds,
+                       (hipStream_t)stream, s->tree, tb, *p, frame, start_block, levels);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
 }
@@ -471,6 +663,7 @@ extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *d
   int rc = dr::check_params(s, p);
   if (rc != DR_OK) return rc;
   if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return dr::fail(DR_ERR_ARG, "dr_backward: null buffers");
+  if (p->plugins != DR_PLUGINS_BENCHMARK) return dr::fail(DR_ERR_ARG, "dr_backward: only the benchmark plugin set has an adjoint");
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const int maxv = p->max_bounces + 1;
   const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
@@ -620,6 +813,8 @@ extern "C" int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tun
   for (int r = 0; r < n && rc == DR_OK; r++) {
     rc = dr::check_params(reqs[r].scene, &reqs[r].params);
     if (rc == DR_OK && !reqs[r].frame) rc = dr::fail(DR_ERR_ARG, "dr_batch_create: null frame");
+    if (rc == DR_OK && reqs[r].params.plugins != DR_PLUGINS_BENCHMARK)
+      rc = dr::fail(DR_ERR_ARG, "dr_batch_create: batches render the benchmark plugin set only");
     if (rc != DR_OK) break;
     dr::DevRequest d;
     d.T = reqs[r].scene->tree;

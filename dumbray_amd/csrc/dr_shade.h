@@ -70,8 +70,9 @@ DR_HD V3 normalized(V3 a) {
 }
 
 // Camera sample of subpixel (i, j) of pixel (px, py): DumbRenderer.cu:190-232 (HEIGHT
-// boxing), SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:12-20 with
-// DumbRand::range = u*(max-min)+min), Camera transform (Transform.impl.h:166-168).
+// boxing), SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21 with
+// DumbRand::range = u*(max-min)+min) or DefaultPerspectiveLense, Camera transform
+// (Transform.impl.h:166-168).
 DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, float u0, float u1, V3 &o, V3 &d) {
   const float W = (float)P.width, H = (float)P.height;
   const float ps = (1.0f / H);
@@ -80,10 +81,18 @@ DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, f
   const float ppy = ((H / 2.0f) - (float)py) * ps;
   const float sx = ppx + (((float)i) + 0.5f) * pdx;
   const float sy = ppy + (((float)j) + 0.5f) * pdy;
-  const float offx = pdx * 0.5f, offy = pdy * 0.5f;
-  const float lx = sx + ((u0 * (offx - (-offx))) + (-offx));
-  const float ly = sy + ((u1 * (offy - (-offy))) + (-offy));
-  const float lz = P.cam[12];
+  float lx, ly, lz;
+  if (P.lens == DR_PERSPECTIVE_LENS) {
+    // DefaultPerspectiveLense (DefaultPerspectiveLense.impl.cuh:12-15): subpixel centre,
+    // direction normalized in lens space before the camera transform
+    const V3 n = normalized(v3(sx, sy, P.cam[12]));
+    lx = n.x; ly = n.y; lz = n.z;
+  } else {
+    const float offx = pdx * 0.5f, offy = pdy * 0.5f;
+    lx = sx + ((u0 * (offx - (-offx))) + (-offx));
+    ly = sy + ((u1 * (offy - (-offy))) + (-offy));
+    lz = P.cam[12];
+  }
   const float *C = P.cam;
   o = v3(C[0], C[1], C[2]);
   d = v3((lx * C[3] + ly * C[6]) + lz * C[9], (lx * C[4] + ly * C[7]) + lz * C[10], (lx * C[5] + ly * C[8]) + lz * C[11]);

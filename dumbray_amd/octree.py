@@ -121,6 +121,18 @@ class DeviceScene:
     def device_bytes(self) -> int:
         return int(lib().dr_scene_device_bytes(self._h))
 
+    def set_normals(self, normals=None):
+        """Per-vertex normals of every triangle (BakedTriFace::norm): (ntris, 9) float32 or
+        None for flat normals, the unit Triangle::normal (c - a) x (b - a)."""
+        if normals is None:
+            check(lib().dr_scene_set_normals(self._h, None, self.ntris), "dr_scene_set_normals")
+            return
+        n = np.ascontiguousarray(np.asarray(normals, np.float32).reshape(-1, 9))
+        if n.shape[0] != self.ntris:
+            raise DumbRayError("set_normals: one row of 9 floats per triangle required")
+        check(lib().dr_scene_set_normals(self._h, n.ctypes.data_as(ctypes.c_void_p), self.ntris),
+              "dr_scene_set_normals")
+
     def cast(self, rays, clip_backfaces: bool = True, exclude=None, points: bool = False, stream=None):
         """Batch closest hit.  rays: torch float32 (n, 6) on the GPU.  Returns dict of
         torch tensors id (int32, -1 = miss), t (float32, FLT_MAX on miss) [, point]."""

@@ -1,0 +1,161 @@
+"""Render plugins: materials, lights and lenses as the device tables store them.
+
+Each constructor mirrors a plugin of the reference and does what its constructor
+does, in fp32, so the table holds the same values the reference object would:
+
+  materials   lambert / mirror / metal          (the benchmark plugins)
+              dumb_based_shader + presets       DumbBasedShader.cuh:14-17, .impl.cuh:5-16
+              simple_stochastic_shader          SimpleStochasticShader.impl.cuh:4-11
+              fresnel_shader                    FresnelShader.impl.cuh:6-9
+              default_shader                    DefaultShader.impl.cuh:5-13
+  lights      area_light                        (the benchmark plugin)
+              directional_light                 SimpleDirectionalLight.impl.cuh:7-11
+              soft_directional_light            SimpleSoftDirectionalLight.impl.cuh:5-10
+              spherical_light                   SphericalLightEmitter.impl.cuh:5-9
+  lenses      STOCHASTIC_LENS (SimpleStochasticLense), PERSPECTIVE_LENS (DefaultPerspectiveLense)
+
+Textures are not supported: every ColoredTexture is its colour.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MATERIAL_KINDS = {"lambert": 0, "mirror": 1, "metal": 2, "dumb_based": 3, "simple_stochastic": 4, "fresnel": 5,
+                  "default": 6}
+LIGHT_KINDS = {"quad": 0, "area": 0, "directional": 1, "soft_directional": 2, "spherical": 3}
+STOCHASTIC_LENS, PERSPECTIVE_LENS = 0, 1
+PLUGINS_BENCHMARK, PLUGINS_REFERENCE = 0, 1
+
+f32 = np.float32
+
+
+def _rgb(c):
+    return tuple(float(f32(x)) for x in c)
+
+
+def _normalized(v):
+    """Vector3::normalized in fp32: v * (1 / sqrt(x*x + y*y + z*z))."""
+    x, y, z = (f32(a) for a in v)
+    inv = f32(1.0) / np.sqrt(f32(f32(x * x + y * y) + z * z), dtype=f32)
+    return (float(x * inv), float(y * inv), float(z * inv))
+
+
+# ---------------------------------------------------------------- materials
+def lambert(albedo):
+    return {"kind": "lambert", "albedo": _rgb(albedo)}
+
+
+def mirror(reflectance):
+    return {"kind": "mirror", "albedo": _rgb(reflectance)}
+
+
+def metal(f0):
+    return {"kind": "metal", "albedo": _rgb(f0)}
+
+
+def dumb_based_shader(fresnel=(0.2, 0.2, 0.2), specular=8.0, diffuse=(0.2, 0.2, 0.2), metal=0.5):
+    """DumbBasedShader(fresnelFactor, cpecular, diffuse, metal): metal clamped to [0, 1],
+    diffuse scaled by (1 - metal)."""
+    m = max(min(f32(metal), f32(1.0)), f32(0.0))
+    d = tuple(float(f32(c) * (f32(1.0) - m)) for c in diffuse)
+    return {"kind": "dumb_based", "albedo": _rgb(fresnel), "diffuse": d, "p": (float(f32(specular)), float(m), 0.0)}
+
+
+def rough_gold():
+    return dumb_based_shader((1.022, 0.782, 0.344), 16.0, (1.0, 1.0, 1.0), 1.0)
+
+
+def glossy_gold():
+    return dumb_based_shader((1.022, 0.782, 0.344), 128.0, (1.0, 1.0, 1.0), 1.0)
+
+
+def glossy_finish():
+    return dumb_based_shader((0.175, 0.15, 0.125), 4096.0, (0.32, 0.32, 0.32), 0.25)
+
+
+def matte_finish():
+    return dumb_based_shader((0.175, 0.15, 0.125), 8.0, (0.24, 0.24, 0.24), 0.05)
+
+
+def _diffuse_gloss(diffuse, smoothness):
+    total = f32(diffuse) + f32(smoothness)
+    mul = (f32(1.0) / total) if total > f32(1.0) else f32(1.0)
+    return float(f32(diffuse) * mul), float(f32(smoothness) * mul)
+
+
+def simple_stochastic_shader(color=(1.0, 1.0, 1.0), diffuse=0.05, smoothness=0.95, shininess=2.0):
+    d, g = _diffuse_gloss(diffuse, smoothness)
+    return {"kind": "simple_stochastic", "albedo": _rgb(color), "p": (d, g, float(f32(shininess)))}
+
+
+def fresnel_shader(color=(1.0, 1.0, 1.0)):
+    return {"kind": "fresnel", "albedo": _rgb(color)}
+
+
+def default_shader(color=(1.0, 1.0, 1.0), diffuse=0.7, smoothness=0.2, shine=8.0):
+    d, g = _diffuse_gloss(diffuse, smoothness)
+    return {"kind": "default", "albedo": _rgb(color), "p": (d, g, float(f32(shine)))}
+
+
+def is_reference_material(m) -> bool:
+    return MATERIAL_KINDS[m["kind"]] >= MATERIAL_KINDS["dumb_based"]
+
+
+# ---------------------------------------------------------------- lights
+def area_light(corner, edge_u, edge_v, emission):
+    """One-sided quad emitting on the (edge_u x edge_v) side."""
+    return {"kind": "quad", "corner": tuple(corner), "edge_u": tuple(edge_u), "edge_v": tuple(edge_v),
+            "emission": _rgb(emission)}
+
+
+def directional_light(color, direction, distance):
+    return {"kind": "directional", "emission": _rgb(color), "edge_u": _normalized(direction),
+            "dist": float(f32(distance))}
+
+
+def soft_directional_light(color=(1.0, 1.0, 1.0), direction=None, distance=1024.0, softness=0.125):
+    if direction is None:
+        direction = _normalized((1.0, -1.0, 1.0))
+    return {"kind": "soft_directional", "emission": _rgb(color), "edge_u": _normalized(direction),
+            "dist": float(f32(distance)), "radius": float(f32(distance) * f32(softness))}
+
+
+def spherical_light(color=(1.0, 1.0, 1.0), position=(0.0, 0.0, 0.0), radius=1.0):
+    return {"kind": "spherical", "emission": _rgb(color), "corner": tuple(position), "radius": float(f32(radius))}
+
+
+def is_reference_light(l) -> bool:
+    return LIGHT_KINDS[l.get("kind", "quad")] != 0
+
+
+# ---------------------------------------------------------------- device tables
+def material_table(materials) -> np.ndarray:
+    """(n, 12) float32 rows in dr_material layout."""
+    t = np.zeros((max(len(materials), 1), 12), np.float32)
+    for i, m in enumerate(materials):
+        t[i, 0] = np.array([MATERIAL_KINDS[m["kind"]]], np.int32).view(np.float32)[0]
+        t[i, 1:4] = m.get("albedo", (0.0, 0.0, 0.0))
+        t[i, 4:7] = m.get("diffuse", (0.0, 0.0, 0.0))
+        t[i, 7:10] = m.get("p", (0.0, 0.0, 0.0))
+    return t
+
+
+def light_table(lights) -> np.ndarray:
+    """(n, 16) float32 rows in dr_light layout."""
+    t = np.zeros((max(len(lights), 1), 16), np.float32)
+    for i, l in enumerate(lights):
+        t[i, 0:3] = l.get("corner", (0.0, 0.0, 0.0))
+        t[i, 3:6] = l.get("edge_u", (0.0, 0.0, 0.0))
+        t[i, 6:9] = l.get("edge_v", (0.0, 0.0, 0.0))
+        t[i, 9:12] = l.get("emission", (0.0, 0.0, 0.0))
+        t[i, 12] = np.array([LIGHT_KINDS[l.get("kind", "quad")]], np.int32).view(np.float32)[0]
+        t[i, 13] = l.get("dist", 0.0)
+        t[i, 14] = l.get("radius", 0.0)
+    return t
+
+
+def plugin_set(materials, lights) -> int:
+    """The plugin set a scene needs: the reference estimator once any reference plugin is used."""
+    if any(is_reference_material(m) for m in materials) or any(is_reference_light(l) for l in lights):
+        return PLUGINS_REFERENCE
+    return PLUGINS_BENCHMARK

@@ -16,11 +16,13 @@ import ctypes
 
 import numpy as np
 
+from . import plugins as PL
 from ._lib import DumbRayError, check, lib
 from .octree import DeviceScene
 
 DR_LAMBERT, DR_MIRROR, DR_METAL = 0, 1, 2
-KINDS = {"lambert": DR_LAMBERT, "mirror": DR_MIRROR, "metal": DR_METAL}
+KINDS = PL.MATERIAL_KINDS
+LENSES = {"stochastic": PL.STOCHASTIC_LENS, "perspective": PL.PERSPECTIVE_LENS}
 BLOCK_W, BLOCK_H = 8, 16
 BLOCK_SIZE = BLOCK_W * BLOCK_H
 DUMB_RENDERER_BOUNCE_LIMIT = 64      # DumbRenderer.cuh:7
@@ -32,7 +34,8 @@ class RenderParams(ctypes.Structure):
                 ("rr_survive", ctypes.c_float), ("cull_backfaces", ctypes.c_int32), ("seed_lo", ctypes.c_uint32),
                 ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
                 ("cam", ctypes.c_float * 13), ("materials", ctypes.c_void_p), ("lights", ctypes.c_void_p),
-                ("nmat", ctypes.c_int32), ("nlights", ctypes.c_int32)]
+                ("nmat", ctypes.c_int32), ("nlights", ctypes.c_int32), ("lens", ctypes.c_int32),
+                ("plugins", ctypes.c_int32)]
 
 
 def _vp(x):
@@ -90,7 +93,8 @@ class BlockBasedFrameBuffer:
 class DumbRenderer:
     def __init__(self, scene: DeviceScene, mat_id, materials, lights, camera, width: int, height: int,
                  samples_per_pixel=(1, 1), max_bounces: int = 2, rr_start: int = 1 << 30, rr_survive: float = 1.0,
-                 ignore_backfaces: bool = True, seed: int = 0x5EED, device="cuda"):
+                 ignore_backfaces: bool = True, seed: int = 0x5EED, device="cuda", lens: str = "stochastic",
+                 normals=None):
         import torch
 
         self.scene = scene
@@ -104,21 +108,18 @@ class DumbRenderer:
         mid = np.ascontiguousarray(np.asarray(mat_id, dtype=np.int32))
         check(lib().dr_scene_set_materials(scene.handle, mid.ctypes.data_as(ctypes.c_void_p), int(mid.shape[0])),
               "dr_scene_set_materials")
-        mt = np.zeros((len(materials), 4), np.float32)
-        for i, m in enumerate(materials):
-            mt[i, 0] = np.array([KINDS[m["kind"]]], np.int32).view(np.float32)[0]
-            mt[i, 1:4] = m["albedo"]
-        lt = np.zeros((max(len(lights), 1), 12), np.float32)
-        for i, l in enumerate(lights):
-            lt[i] = list(l["corner"]) + list(l["edge_u"]) + list(l["edge_v"]) + list(l["emission"])
-        self.material_table = torch.from_numpy(mt).to(device)
-        self.light_table = torch.from_numpy(lt).to(device)
+        if normals is not None:
+            scene.set_normals(normals)
+        self.lens = LENSES[lens]
+        self.plugins = PL.plugin_set(materials, lights)
+        self.material_table = torch.from_numpy(PL.material_table(materials)).to(device)
+        self.light_table = torch.from_numpy(PL.light_table(lights)).to(device)
         self.nmat, self.nlights = len(materials), len(lights)
 
     # parameters an optimiser may update in place
     @property
     def albedo(self):
-        return self.material_table[:, 1:4]
+        return self.material_table[: self.nmat, 1:4]
 
     @property
     def emission(self):
@@ -138,6 +139,7 @@ class DumbRenderer:
         p.materials = self.material_table.data_ptr()
         p.lights = self.light_table.data_ptr()
         p.nmat, p.nlights = self.nmat, self.nlights
+        p.lens, p.plugins = self.lens, self.plugins
         return p
 
     def renderBlocksGPU(self, frame: BlockBasedFrameBuffer, start_block: int, end_block: int, iteration: int = 1,

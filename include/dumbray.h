@@ -78,20 +78,67 @@ int dr_cast(dr_scene s, const float *rays, int n, unsigned flags, const int32_t 
 
 /* ---------------------------------------------------------------- path tracing */
 
-/* Material kinds (plugins of Material<BakedTriFace>, Material.cuh:115-123). */
-enum { DR_LAMBERT = 0, DR_MIRROR = 1, DR_METAL = 2 };
+/* Material kinds (plugins of Material<BakedTriFace>, Material.cuh:115-123).  Kinds 0-2
+ * are the plugins the benchmark scenes use (the reference has no such shaders); kinds
+ * 3-6 are the reference's own shaders, without textures (ColoredTexture colours only). */
+enum {
+  DR_LAMBERT = 0,
+  DR_MIRROR = 1,
+  DR_METAL = 2,
+  DR_DUMB_BASED = 3,          /* DumbBasedShader, DumbBasedShader.impl.cuh:5-122 */
+  DR_SIMPLE_STOCHASTIC = 4,   /* SimpleStochasticShader, SimpleStochasticShader.impl.cuh:4-37 */
+  DR_FRESNEL = 5,             /* Shaders::FresnelShader, FresnelShader.impl.cuh:6-19 */
+  DR_DEFAULT_SHADER = 6       /* DefaultShaderGeneric, DefaultShader.impl.cuh:5-52 */
+};
 
-/* One material: Lambert albedo / mirror reflectance / Schlick metal F0 (16 B). */
+/* One material (48 B).  Fields by kind:
+ *   LAMBERT / MIRROR / METAL: albedo = Lambert albedo / mirror reflectance / Schlick F0.
+ *   DUMB_BASED: albedo = fresnel colour, diffuse = diffuse colour already scaled by
+ *     (1 - metal) as the constructor does, p[0] = specular exponent, p[1] = metal in [0, 1].
+ *   SIMPLE_STOCHASTIC / DEFAULT_SHADER: albedo = colour, p[0] = diffuse, p[1] = gloss
+ *     (both divided by their sum when it exceeds 1, as the constructors do), p[2] = shininess.
+ *   FRESNEL: albedo = fresnel colour. */
 typedef struct {
   int32_t kind;
   float albedo[3];
+  float diffuse[3];
+  float p[3];
+  int32_t reserved[2];
 } dr_material;
 
-/* One-sided area quad light emitting on the (edge_u x edge_v) side (48 B);
- * the Light plugin interface is Light::getVertexPhotons (Light.cuh:53-59). */
+/* Light kinds (Light::getVertexPhotons, Light.impl.cuh:34-36).  AREA is the benchmark
+ * plugin; the others are the reference's own lights. */
+enum {
+  DR_AREA_LIGHT = 0,              /* one-sided quad emitting on the (edge_u x edge_v) side */
+  DR_DIRECTIONAL_LIGHT = 1,       /* SimpleDirectionalLight, SimpleDirectionalLight.impl.cuh:7-16 */
+  DR_SOFT_DIRECTIONAL_LIGHT = 2,  /* SimpleSoftDirectionalLight, SimpleSoftDirectionalLight.impl.cuh:5-19 */
+  DR_SPHERICAL_LIGHT = 3          /* SphericalLightEmitter, SphericalLightEmitter.impl.cuh:5-18 */
+};
+
+/* One light (64 B).  Fields by kind:
+ *   AREA: corner, edge_u, edge_v, emission (radiance scale).
+ *   DIRECTIONAL: emission = colour, edge_u = unit direction, dist = distance of the photon origin.
+ *   SOFT_DIRECTIONAL: as DIRECTIONAL, plus radius = distance * softness.
+ *   SPHERICAL: emission = colour, corner = centre, radius. */
 typedef struct {
   float corner[3], edge_u[3], edge_v[3], emission[3];
+  int32_t kind;
+  float dist, radius;
+  int32_t reserved;
 } dr_light;
+
+/* Lenses (Lense plugins, Lense.cuh:46-82). */
+enum {
+  DR_STOCHASTIC_LENS = 0,  /* SimpleStochasticLense: jitter inside the subpixel, SimpleStochasticLense.impl.cuh:11-25 */
+  DR_PERSPECTIVE_LENS = 1  /* DefaultPerspectiveLense: subpixel centre, unit direction, DefaultPerspectiveLense.impl.cuh:12-19 */
+};
+
+/* Plugin sets. */
+enum {
+  DR_PLUGINS_BENCHMARK = 0,  /* kinds 0-2 and area lights; blockers are opaque */
+  DR_PLUGINS_REFERENCE = 1   /* every kind above and the reference's photon transparency
+                                (DumbRenderer.cu:396-413); dr_render_blocks only */
+};
 
 /* Per-call render parameters (DumbRenderer::PixelRenderProcess::SceneConfiguration,
  * DumbRenderer.cuh:80-101, plus the estimator's seeds).  Boxing is HEIGHT based. */
@@ -109,11 +156,19 @@ typedef struct {
   const dr_material *materials;   /* device table [nmat] */
   const dr_light *lights;         /* device table [nlights] */
   int32_t nmat, nlights;          /* nlights <= 16 */
+  int32_t lens;                   /* DR_STOCHASTIC_LENS or DR_PERSPECTIVE_LENS */
+  int32_t plugins;                /* DR_PLUGINS_BENCHMARK or DR_PLUGINS_REFERENCE */
 } dr_render_params;
 
 /* Attach per-triangle material ids (host int32[ntris], ntris = the octree's triangle
  * count; the scene keeps a device copy).  Needed before rendering. */
 int dr_scene_set_materials(dr_scene s, const int32_t *mat_id, int ntris);
+
+/* Per-vertex normals of every triangle (BakedTriFace::norm, BakedTriMesh.h:8): host
+ * float[ntris * 9] (a, b, c normals), or NULL for flat normals, the unit
+ * (c - a) x (b - a) of each triangle (Triangle::normal, Triangle.impl.h:233-235), which
+ * is also the default.  Used by the reference shaders (DR_PLUGINS_REFERENCE). */
+int dr_scene_set_normals(dr_scene s, const float *normals, int ntris);
 
 /* DumbRenderer::renderBlocksGPU (DumbRenderer.cu:87-113, override of
  * BlockRenderer::renderBlocksGPU, BlockRenderer.cuh:43-44): renders framebuffer

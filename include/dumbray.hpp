@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>
@@ -84,6 +85,15 @@ class Octree {
     std::vector<int32_t> mat(data_.size());
     for (size_t i = 0; i < data_.size(); i++) mat[i] = data_[i].materialId;
     check(dr_scene_set_materials(scene_, mat.data(), (int)mat.size()), "dr_scene_set_materials");
+    // BakedTriFace::norm, when the faces carry vertex normals (all-zero normals mean flat shading)
+    std::vector<float> nrm(9 * data_.size());
+    bool any = false;
+    for (size_t i = 0; i < data_.size(); i++) {
+      const Triangle &n = data_[i].object.norm;
+      const float v[9] = {n.a.x, n.a.y, n.a.z, n.b.x, n.b.y, n.b.z, n.c.x, n.c.y, n.c.z};
+      for (int k = 0; k < 9; k++) { nrm[9 * i + k] = v[k]; any = any || v[k] != 0.0f; }
+    }
+    if (any) check(dr_scene_set_normals(scene_, nrm.data(), (int)data_.size()), "dr_scene_set_normals");
   }
   int getNodeCount() const {
     int n = 0;
@@ -168,6 +178,98 @@ class BlockBasedFrameBuffer {
 };
 typedef BlockBasedFrameBuffer<8, 16> BlockBuffer;
 
+// Render plugin constructors: each does what the reference plugin's constructor does, in
+// fp32, so the device table holds the values the reference object would.  Textures are
+// not supported (every ColoredTexture is its colour).
+namespace Plugins {
+inline dr_material material(int32_t kind, Vector3 albedo) {
+  dr_material m{};
+  m.kind = kind;
+  m.albedo[0] = albedo.x; m.albedo[1] = albedo.y; m.albedo[2] = albedo.z;
+  return m;
+}
+inline dr_material lambert(Vector3 albedo) { return material(DR_LAMBERT, albedo); }
+inline dr_material mirror(Vector3 reflectance) { return material(DR_MIRROR, reflectance); }
+inline dr_material metal(Vector3 f0) { return material(DR_METAL, f0); }
+// DumbBasedShader (DumbBasedShader.impl.cuh:5-16): metal clamped to [0, 1], diffuse *= 1 - metal.
+inline dr_material dumbBasedShader(Vector3 fresnel = {0.2f, 0.2f, 0.2f}, float specular = 8.0f,
+                                   Vector3 diffuse = {0.2f, 0.2f, 0.2f}, float metalness = 0.5f) {
+  dr_material m = material(DR_DUMB_BASED, fresnel);
+  const float mt = std::max(std::min(metalness, 1.0f), 0.0f);
+  m.diffuse[0] = diffuse.x * (1.0f - mt); m.diffuse[1] = diffuse.y * (1.0f - mt); m.diffuse[2] = diffuse.z * (1.0f - mt);
+  m.p[0] = specular;
+  m.p[1] = mt;
+  return m;
+}
+// Presets, DumbBasedShader.cuh:14-17.
+inline dr_material roughGold() { return dumbBasedShader({1.022f, 0.782f, 0.344f}, 16.0f, {1.0f, 1.0f, 1.0f}, 1.0f); }
+inline dr_material glossyGold() { return dumbBasedShader({1.022f, 0.782f, 0.344f}, 128.0f, {1.0f, 1.0f, 1.0f}, 1.0f); }
+inline dr_material glossyFinish() { return dumbBasedShader({0.175f, 0.15f, 0.125f}, 4096.0f, {0.32f, 0.32f, 0.32f}, 0.25f); }
+inline dr_material matteFinish() { return dumbBasedShader({0.175f, 0.15f, 0.125f}, 8.0f, {0.24f, 0.24f, 0.24f}, 0.05f); }
+// diffuse and gloss divided by their sum when it exceeds 1 (SimpleStochasticShader.impl.cuh:4-11,
+// DefaultShader.impl.cuh:5-13).
+inline dr_material diffuseGloss(int32_t kind, Vector3 color, float diffuse, float smoothness, float shininess) {
+  dr_material m = material(kind, color);
+  const float total = diffuse + smoothness;
+  const float mul = (total > 1.0f) ? (1.0f / total) : 1.0f;
+  m.p[0] = diffuse * mul;
+  m.p[1] = smoothness * mul;
+  m.p[2] = shininess;
+  return m;
+}
+inline dr_material simpleStochasticShader(Vector3 color = {1.0f, 1.0f, 1.0f}, float diffuse = 0.05f,
+                                          float smoothness = 0.95f, float shininess = 2.0f) {
+  return diffuseGloss(DR_SIMPLE_STOCHASTIC, color, diffuse, smoothness, shininess);
+}
+inline dr_material fresnelShader(Vector3 color = {1.0f, 1.0f, 1.0f}) { return material(DR_FRESNEL, color); }
+inline dr_material defaultShader(Vector3 color = {1.0f, 1.0f, 1.0f}, float diffuse = 0.7f, float smoothness = 0.2f,
+                                 float shine = 8.0f) {
+  return diffuseGloss(DR_DEFAULT_SHADER, color, diffuse, smoothness, shine);
+}
+inline Vector3 normalized(Vector3 v) {   // Vector3::normalized
+  const float inv = 1.0f / std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  return {v.x * inv, v.y * inv, v.z * inv};
+}
+inline dr_light areaLight(Vector3 corner, Vector3 edgeU, Vector3 edgeV, Vector3 emission) {
+  dr_light l{};
+  l.kind = DR_AREA_LIGHT;
+  const float v[12] = {corner.x, corner.y, corner.z, edgeU.x, edgeU.y, edgeU.z,
+                       edgeV.x,  edgeV.y,  edgeV.z,  emission.x, emission.y, emission.z};
+  for (int k = 0; k < 3; k++) {
+    l.corner[k] = v[k]; l.edge_u[k] = v[3 + k]; l.edge_v[k] = v[6 + k]; l.emission[k] = v[9 + k];
+  }
+  return l;
+}
+// SimpleDirectionalLight (SimpleDirectionalLight.impl.cuh:7-11)
+inline dr_light directionalLight(Vector3 color, Vector3 direction, float distance) {
+  dr_light l{};
+  l.kind = DR_DIRECTIONAL_LIGHT;
+  const Vector3 d = normalized(direction);
+  l.edge_u[0] = d.x; l.edge_u[1] = d.y; l.edge_u[2] = d.z;
+  l.emission[0] = color.x; l.emission[1] = color.y; l.emission[2] = color.z;
+  l.dist = distance;
+  return l;
+}
+// SimpleSoftDirectionalLight (SimpleSoftDirectionalLight.impl.cuh:5-10)
+inline dr_light softDirectionalLight(Vector3 color = {1.0f, 1.0f, 1.0f}, Vector3 direction = {1.0f, -1.0f, 1.0f},
+                                     float distance = 1024.0f, float softness = 0.125f) {
+  dr_light l = directionalLight(color, direction, distance);
+  l.kind = DR_SOFT_DIRECTIONAL_LIGHT;
+  l.radius = distance * softness;
+  return l;
+}
+// SphericalLightEmitter (SphericalLightEmitter.impl.cuh:5-9)
+inline dr_light sphericalLight(Vector3 color = {1.0f, 1.0f, 1.0f}, Vector3 position = {0.0f, 0.0f, 0.0f},
+                               float radius = 1.0f) {
+  dr_light l{};
+  l.kind = DR_SPHERICAL_LIGHT;
+  l.corner[0] = position.x; l.corner[1] = position.y; l.corner[2] = position.z;
+  l.emission[0] = color.x; l.emission[1] = color.y; l.emission[2] = color.z;
+  l.radius = radius;
+  return l;
+}
+}  // namespace Plugins
+
 // Camera = lens focal length + Transform (position and world axes).
 struct Camera {
   Vector3 position, right, up, forward;
@@ -194,6 +296,12 @@ class DumbRenderer {
     setMaxBounces(maxBounces);
     setSamplesPerPixel(samplesPerPixelX, samplesPerPixelY);
     ignoreBackfaces_ = ignoreBackfaces;
+    // the reference estimator as soon as any of the reference's own plugins is used
+    plugins_ = DR_PLUGINS_BENCHMARK;
+    for (const dr_material &m : materials)
+      if (m.kind >= DR_DUMB_BASED) plugins_ = DR_PLUGINS_REFERENCE;
+    for (const dr_light &l : lights)
+      if (l.kind != DR_AREA_LIGHT) plugins_ = DR_PLUGINS_REFERENCE;
   }
   ~DumbRenderer() {
     (void)hipFree(mats_);
@@ -212,6 +320,7 @@ class DumbRenderer {
     rrSurvive_ = survive;
   }
   void setSeed(uint64_t seed) { seed_ = seed; }
+  void setLens(int lens) { lens_ = lens; }   // DR_STOCHASTIC_LENS or DR_PERSPECTIVE_LENS
   dr_material *deviceMaterials() { return mats_; }
   dr_light *deviceLights() { return lights_; }
 
@@ -238,6 +347,8 @@ class DumbRenderer {
     p.lights = lights_;
     p.nmat = nmat_;
     p.nlights = nlights_;
+    p.lens = lens_;
+    p.plugins = plugins_;
     return p;
   }
   // DumbRenderer::renderBlocksGPU: blocks [startBlock, endBlock) of `iteration` into the device frame.
@@ -266,6 +377,7 @@ class DumbRenderer {
   float rrSurvive_ = 1.0f;
   bool ignoreBackfaces_ = true;
   uint64_t seed_ = 0x5EED;
+  int lens_ = DR_STOCHASTIC_LENS, plugins_ = DR_PLUGINS_BENCHMARK;
 };
 
 }  // namespace dumbray

@@ -69,12 +69,14 @@ def lib():
 
 
 class OraMaterial(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int32), ("albedo", ctypes.c_float * 3)]
+    _fields_ = [("kind", ctypes.c_int32), ("albedo", ctypes.c_float * 3), ("diffuse", ctypes.c_float * 3),
+                ("p", ctypes.c_float * 3), ("reserved", ctypes.c_int32 * 2)]
 
 
 class OraLight(ctypes.Structure):
     _fields_ = [("corner", ctypes.c_float * 3), ("edge_u", ctypes.c_float * 3), ("edge_v", ctypes.c_float * 3),
-                ("emission", ctypes.c_float * 3)]
+                ("emission", ctypes.c_float * 3), ("kind", ctypes.c_int32), ("dist", ctypes.c_float),
+                ("radius", ctypes.c_float), ("reserved", ctypes.c_int32)]
 
 
 class OraRenderCfg(ctypes.Structure):
@@ -82,10 +84,37 @@ class OraRenderCfg(ctypes.Structure):
                 ("fsaa_y", ctypes.c_int32), ("max_bounces", ctypes.c_int32), ("rr_start", ctypes.c_int32),
                 ("rr_survive", ctypes.c_float), ("cull_backfaces", ctypes.c_int32), ("seed_lo", ctypes.c_uint32),
                 ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
-                ("cam", ctypes.c_float * 13)]
+                ("cam", ctypes.c_float * 13), ("lens", ctypes.c_int32)]
 
 
-KINDS = {"lambert": 0, "mirror": 1, "metal": 2}
+KINDS = {"lambert": 0, "mirror": 1, "metal": 2, "dumb_based": 3, "simple_stochastic": 4, "fresnel": 5,
+         "default": 6}
+LIGHT_KINDS = {"quad": 0, "area": 0, "directional": 1, "soft_directional": 2, "spherical": 3}
+
+
+def pack_material(m, out=None):
+    """OraMaterial from a material dict: kind, albedo and the optional diffuse / p fields."""
+    r = out if out is not None else OraMaterial()
+    r.kind = KINDS[m["kind"]]
+    for c in range(3):
+        r.albedo[c] = float(np.float32(m.get("albedo", (0, 0, 0))[c]))
+        r.diffuse[c] = float(np.float32(m.get("diffuse", (0, 0, 0))[c]))
+        r.p[c] = float(np.float32(m.get("p", (0, 0, 0))[c]))
+    return r
+
+
+def pack_light(l, out=None):
+    """OraLight from a light dict: kind (default quad) and its fields."""
+    r = out if out is not None else OraLight()
+    r.kind = LIGHT_KINDS[l.get("kind", "quad")]
+    for c in range(3):
+        r.corner[c] = l.get("corner", (0, 0, 0))[c]
+        r.edge_u[c] = l.get("edge_u", (0, 0, 0))[c]
+        r.edge_v[c] = l.get("edge_v", (0, 0, 0))[c]
+        r.emission[c] = l.get("emission", (0, 0, 0))[c]
+    r.dist = float(l.get("dist", 0.0))
+    r.radius = float(l.get("radius", 0.0))
+    return r
 
 
 def _bind_render(L):
@@ -95,8 +124,81 @@ def _bind_render(L):
                              ctypes.c_int]
     L.ora_backward.restype = ctypes.c_int
     L.ora_backward.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
+    L.ora_render_ref.restype = ctypes.c_int
+    L.ora_render_ref.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int]
     L.ora_philox.argtypes = [vp, vp, vp]
     L.ora_sincos_2pi.argtypes = [ctypes.c_float, vp, vp]
+    L.ora_pow.restype = ctypes.c_float
+    L.ora_pow.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.ora_point_on_sphere.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]
+    L.ora_reflected_color.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
+    L.ora_indirect_sample.restype = ctypes.c_int
+    L.ora_indirect_sample.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp, vp]
+    L.ora_light_photon.restype = ctypes.c_int
+    L.ora_light_photon.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp, vp,
+                                   vp]
+
+
+def _flat32(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1))
+    assert n is None or a.size == n
+    return a
+
+
+def pow_f(x, y):
+    """The oracle's pow (the reference shaders' pow calls)."""
+    return lib().ora_pow(ctypes.c_float(x), ctypes.c_float(y))
+
+
+def point_on_sphere(ua, ub, radius=1.0):
+    out = np.zeros(3, np.float32)
+    lib().ora_point_on_sphere(ctypes.c_float(ua), ctypes.c_float(ub), ctypes.c_float(radius), out.ctypes.data)
+    return out
+
+
+def reflected_color(material, tri, normals, p, photon_dir, photon_color, observer, photon_type, sample_type=0):
+    """getReflectedColor of a reference shader (photon_type 0 direct, 1 indirect); normals None = flat."""
+    m = pack_material(material)
+    t, pp, pd, pc, ob = _flat32(tri, 9), _flat32(p, 3), _flat32(photon_dir, 3), _flat32(photon_color, 3), _flat32(observer, 3)
+    nr = None if normals is None else _flat32(normals, 9)
+    out = np.zeros(3, np.float32)
+    lib().ora_reflected_color(ctypes.addressof(m), t.ctypes.data, None if nr is None else nr.ctypes.data,
+                              pp.ctypes.data, pd.ctypes.data, pc.ctypes.data, ob.ctypes.data, int(photon_type),
+                              int(sample_type), out.ctypes.data)
+    return out
+
+
+def _cfg(seed=0x5EED, request=0):
+    cfg = OraRenderCfg()
+    cfg.seed_lo, cfg.seed_hi = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+    cfg.request = request
+    return cfg
+
+
+def indirect_sample(material, tri, normals, p, d, pix=0, samp=0, layer=0, seed=0x5EED):
+    """requestIndirectSamples of a reference shader: (direction, sample type) or None."""
+    m = pack_material(material)
+    t, pp, dd = _flat32(tri, 9), _flat32(p, 3), _flat32(d, 3)
+    nr = None if normals is None else _flat32(normals, 9)
+    cfg = _cfg(seed)
+    out = np.zeros(3, np.float32)
+    ty = ctypes.c_int()
+    ok = lib().ora_indirect_sample(ctypes.addressof(m), t.ctypes.data, None if nr is None else nr.ctypes.data,
+                                   pp.ctypes.data, dd.ctypes.data, ctypes.addressof(cfg), pix, samp, layer,
+                                   out.ctypes.data, ctypes.byref(ty))
+    return (out, ty.value) if ok else None
+
+
+def light_photon(light, p, index=0, pix=0, samp=0, layer=0, seed=0x5EED):
+    """getVertexPhotons of a light toward p: (origin, direction, colour) or None."""
+    L = pack_light(light)
+    pp = _flat32(p, 3)
+    cfg = _cfg(seed)
+    o, d, c = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
+    ok = lib().ora_light_photon(ctypes.addressof(L), int(index), pp.ctypes.data, ctypes.addressof(cfg), pix, samp,
+                                layer, o.ctypes.data, d.ctypes.data, c.ctypes.data)
+    return (o, d, c) if ok else None
 
 
 def philox(ctr, key):
@@ -118,23 +220,19 @@ class Renderer:
     """Oracle path estimator over an oracle Octree (same settings as dumbray_amd.DumbRenderer)."""
 
     def __init__(self, tree, mat_id, materials, lights, cam_array, width, height, samples_per_pixel=(1, 1),
-                 max_bounces=2, rr_start=1 << 30, rr_survive=1.0, ignore_backfaces=True, seed=0x5EED):
+                 max_bounces=2, rr_start=1 << 30, rr_survive=1.0, ignore_backfaces=True, seed=0x5EED, lens=0,
+                 normals=None):
         self.tree = tree
         self.mat_id = np.ascontiguousarray(mat_id, np.int32)
         self.mats = (OraMaterial * max(len(materials), 1))()
         for i, m in enumerate(materials):
-            self.mats[i].kind = KINDS[m["kind"]]
-            for c in range(3):
-                self.mats[i].albedo[c] = float(np.float32(m["albedo"][c]))
+            pack_material(m, self.mats[i])
         self.nmat = len(materials)
         self.lights = (OraLight * max(len(lights), 1))()
         for i, l in enumerate(lights):
-            for c in range(3):
-                self.lights[i].corner[c] = l["corner"][c]
-                self.lights[i].edge_u[c] = l["edge_u"][c]
-                self.lights[i].edge_v[c] = l["edge_v"][c]
-                self.lights[i].emission[c] = l["emission"][c]
+            pack_light(l, self.lights[i])
         self.nlights = len(lights)
+        self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1)
         cfg = OraRenderCfg()
         cfg.width, cfg.height = int(width), int(height)
         cfg.fsaa_x, cfg.fsaa_y = int(samples_per_pixel[0]), int(samples_per_pixel[1])
@@ -144,6 +242,7 @@ class Renderer:
         cfg.request, cfg.iteration = 0, 1
         for k in range(13):
             cfg.cam[k] = float(cam_array[k])
+        cfg.lens = int(lens)
         self.cfg = cfg
 
     def set_albedo(self, i, rgb):
@@ -165,6 +264,20 @@ class Renderer:
                               ctypes.addressof(self.lights), self.nlights, ctypes.addressof(self.cfg),
                               None if fp64 else out.ctypes.data, out.ctypes.data if fp64 else None, r0, r1,
                               int(nthreads))
+        assert rc == 0
+        return out
+
+    def render_ref(self, iteration=1, nthreads=0, request=0, rows=None):
+        """Image of the reference estimator with the reference's own plugins (ora_render_ref)."""
+        self.cfg.iteration = int(iteration)
+        self.cfg.request = int(request)
+        H, W = self.cfg.height, self.cfg.width
+        r0, r1 = (0, H) if rows is None else (int(rows[0]), int(rows[1]))
+        out = np.zeros((r1 - r0, W, 3), np.float32)
+        nrm = None if self.normals is None else self.normals.ctypes.data
+        rc = lib().ora_render_ref(self.tree.h, self.mat_id.ctypes.data, ctypes.addressof(self.mats), self.nmat,
+                                  ctypes.addressof(self.lights), self.nlights, nrm, ctypes.addressof(self.cfg),
+                                  out.ctypes.data, r0, r1, int(nthreads))
         assert rc == 0
         return out
 

@@ -65,6 +65,7 @@ void ora_sincos_2pi(float u, float *s, float *c) {
 typedef struct {
 	const ora_octree *tree; const int32_t *mat_id; const ora_material *mats; int nmat;
 	const ora_light *lights; int nlights; const ora_render_cfg *cfg;
+	const float *normals;           /* reference estimator: per-vertex normals or NULL */
 } rctx;
 
 /* 4 uniforms in [0,1) for (pixel, sample, layer, purpose): purpose 0 = lens,
@@ -78,9 +79,9 @@ static void rng4(const rctx *C, uint32_t pix, uint32_t samp, uint32_t layer, uin
 }
 
 /* Camera sample of subpixel (i, j) of pixel (px, py): DumbRenderer.cu:190-232 with
- * HEIGHT boxing, SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:12-20,
- * DumbRand::range = u*(max-min)+min), Camera transform (Camera.impl.cuh:7-10,
- * Transform::direction Transform.impl.h:166-168). */
+ * HEIGHT boxing, SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21,
+ * DumbRand::range = u*(max-min)+min) or DefaultPerspectiveLense, Camera transform
+ * (Camera.impl.cuh:7-10, Transform::direction Transform.impl.h:166-168). */
 static void camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, const float u[2], v3 *o, v3 *d) {
 	const float W = (float)c->width, H = (float)c->height;
 	const float ps = (1.0f / H);
@@ -89,10 +90,18 @@ static void camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, co
 	const float ppy = ((H / 2.0f) - (float)py) * ps;
 	const float sx = ppx + (((float)i) + 0.5f) * pdx;
 	const float sy = ppy + (((float)j) + 0.5f) * pdy;
-	const float offx = pdx * 0.5f, offy = pdy * 0.5f;
-	const float lx = sx + ((u[0] * (offx - (-offx))) + (-offx));
-	const float ly = sy + ((u[1] * (offy - (-offy))) + (-offy));
-	const float lz = c->cam[12];
+	float lx, ly, lz;
+	if (c->lens == 1) {
+		/* DefaultPerspectiveLense (DefaultPerspectiveLense.impl.cuh:12-15): the subpixel centre,
+		 * direction normalized in lens space before the camera transform */
+		const v3 n = vnormalized(V(sx, sy, c->cam[12]));
+		lx = n.x; ly = n.y; lz = n.z;
+	} else {
+		const float offx = pdx * 0.5f, offy = pdy * 0.5f;
+		lx = sx + ((u[0] * (offx - (-offx))) + (-offx));
+		ly = sy + ((u[1] * (offy - (-offy))) + (-offy));
+		lz = c->cam[12];
+	}
 	const float *P = c->cam, *R = c->cam + 3, *U = c->cam + 6, *F = c->cam + 9;
 	*o = V(P[0], P[1], P[2]);
 	*d = V((lx * R[0] + ly * U[0]) + lz * F[0], (lx * R[1] + ly * U[1]) + lz * F[1], (lx * R[2] + ly * U[2]) + lz * F[2]);
@@ -193,6 +202,8 @@ static int photon_lit(const rctx *C, v3 q, v3 w, int32_t h, v3 p) {
 #undef REAL
 #undef FN
 
+#include "dr_render_ref.inc"
+
 /* ------------------------------------------------------------------ */
 /* threading                                                           */
 /* ------------------------------------------------------------------ */
@@ -208,6 +219,7 @@ static void *render_worker(void *arg) {
 			const size_t k = (size_t)(py - J->ybase) * cfg->width + px;
 			if (J->mode == 0) pixel_color_f(C, px, py, J->out + 3 * k);
 			else if (J->mode == 1) pixel_color_d(C, px, py, J->out64 + 3 * k);
+			else if (J->mode == 3) pixel_color_ref(C, px, py, J->out + 3 * k);
 			else pixel_backward_d(C, px, py, J->dl + 3 * k, J->ga, J->ge);
 		}
 	return NULL;
@@ -248,7 +260,7 @@ int ora_render(const ora_octree *t, const int32_t *mat_id, const ora_material *m
                int nthreads) {
 	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
 	if (row0 < 0 || row1 > cfg->height || row0 > row1) return 1;
-	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg};
+	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, NULL};
 	return run(&C, out_rgb64 ? 1 : 0, out_rgb, out_rgb64, NULL, NULL, NULL, nthreads, row0, row1);
 }
 
@@ -256,8 +268,17 @@ int ora_backward(const ora_octree *t, const int32_t *mat_id, const ora_material 
                  const ora_light *lights, int nlights, const ora_render_cfg *cfg, const float *dl_dimage,
                  double *g_albedo, double *g_emission, int nthreads) {
 	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
-	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg};
+	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, NULL};
 	memset(g_albedo, 0, sizeof(double) * nmat * 3);
 	memset(g_emission, 0, sizeof(double) * nlights * 3);
 	return run(&C, 2, NULL, NULL, dl_dimage, g_albedo, g_emission, nthreads, 0, cfg->height);
+}
+
+int ora_render_ref(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
+                   const ora_light *lights, int nlights, const float *normals, const ora_render_cfg *cfg,
+                   float *out_rgb, int row0, int row1, int nthreads) {
+	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
+	if (row0 < 0 || row1 > cfg->height || row0 > row1) return 1;
+	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, normals};
+	return run(&C, 3, out_rgb, NULL, NULL, NULL, NULL, nthreads, row0, row1);
 }

@@ -27,15 +27,27 @@
 extern "C" {
 #endif
 
-enum { ORA_LAMBERT = 0, ORA_MIRROR = 1, ORA_METAL = 2 };
+enum { ORA_LAMBERT = 0, ORA_MIRROR = 1, ORA_METAL = 2, ORA_DUMB_BASED = 3, ORA_SIMPLE_STOCHASTIC = 4,
+       ORA_FRESNEL = 5, ORA_DEFAULT_SHADER = 6 };
 
+/* Same layout as the library's dr_material (48 B). */
 typedef struct {
 	int32_t kind;
-	float albedo[3];    /* Lambert albedo, mirror reflectance, metal F0 */
+	float albedo[3];    /* Lambert albedo, mirror reflectance, metal F0; reference shaders: colour / fresnel colour */
+	float diffuse[3];   /* DumbBasedShader diffuse colour (already scaled by 1 - metal) */
+	float p[3];         /* DumbBased: specular, metal; SimpleStochastic / Default: diffuse, gloss, shininess */
+	int32_t reserved[2];
 } ora_material;
 
+enum { ORA_AREA_LIGHT = 0, ORA_DIRECTIONAL_LIGHT = 1, ORA_SOFT_DIRECTIONAL_LIGHT = 2, ORA_SPHERICAL_LIGHT = 3 };
+
+/* Same layout as the library's dr_light (64 B). */
 typedef struct {
-	float corner[3], edge_u[3], edge_v[3], emission[3];   /* emits on the (edge_u x edge_v) side */
+	float corner[3], edge_u[3], edge_v[3], emission[3];   /* area: emits on the (edge_u x edge_v) side */
+	int32_t kind;
+	float dist, radius;     /* directional: edge_u = unit direction, dist; soft: radius = dist * softness;
+	                           spherical: corner = centre, radius */
+	int32_t reserved;
 } ora_light;
 
 typedef struct {
@@ -45,6 +57,7 @@ typedef struct {
 	uint32_t seed_lo, seed_hi, request;
 	int32_t iteration;              /* 1-based; samples of iteration k are numbered from (k-1)*fsaa_x*fsaa_y */
 	float cam[13];                  /* position, right, up, forward, focal (0.5 / tan(fov/2)) */
+	int32_t lens;                   /* 0: SimpleStochasticLense jitter, 1: DefaultPerspectiveLense */
 } ora_render_cfg;
 
 /* Pixel colours (row-major, rows [row0, row1) x W x 3) of one iteration, into out_rgb
@@ -59,6 +72,33 @@ int ora_render(const ora_octree *t, const int32_t *mat_id, const ora_material *m
 int ora_backward(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
                  const ora_light *lights, int nlights, const ora_render_cfg *cfg, const float *dl_dimage,
                  double *g_albedo, double *g_emission, int nthreads);
+
+/* The reference estimator with the reference's own plugins (DumbRenderer.cu:256-443 with
+ * one sample per layer): shaders DumbBasedShader, SimpleStochasticShader, FresnelShader,
+ * DefaultShaderGeneric (no textures), lights SimpleDirectionalLight,
+ * SimpleSoftDirectionalLight, SphericalLightEmitter (and the area quad), and a light
+ * photon that reaches another surface continuing through it with the colour that
+ * surface's getReflectedColor returns when that colour is not black (:396-413).
+ * normals: float[ntris*9] per-vertex normals or NULL for flat unit (c-a)x(b-a).
+ * Layer colours are accumulated bottom-up in fp32. */
+int ora_render_ref(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
+                   const ora_light *lights, int nlights, const float *normals, const ora_render_cfg *cfg,
+                   float *out_rgb, int row0, int row1, int nthreads);
+
+/* Probes of the reference plugins for tests.  photon_type: 0 direct, 1 indirect. */
+float ora_pow(float x, float y);
+void ora_point_on_sphere(float ua, float ub, float radius, float out[3]);
+void ora_reflected_color(const ora_material *m, const float tri[9], const float nrm[9], const float p[3],
+                         const float photon_dir[3], const float photon_color[3], const float observer[3],
+                         int photon_type, int sample_type, float out[3]);
+/* requestIndirectSamples of one material at (tri, p) for the layer ray direction d, with the
+ * uniforms of (pix, samp, layer); returns 1 with the sample direction and type, or 0. */
+int ora_indirect_sample(const ora_material *m, const float tri[9], const float nrm[9], const float p[3],
+                        const float d[3], const ora_render_cfg *cfg, uint32_t pix, uint32_t samp, int layer,
+                        float dir[3], int *type);
+/* getVertexPhotons of light l for the point p; returns 1 with the photon ray and colour, or 0. */
+int ora_light_photon(const ora_light *L, int l, const float p[3], const ora_render_cfg *cfg, uint32_t pix,
+                     uint32_t samp, int layer, float origin[3], float dir[3], float color[3]);
 
 /* Philox4x32-10 block and the sin/cos of 2*pi*u used by the samplers (probes for tests). */
 void ora_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

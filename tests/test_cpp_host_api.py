@@ -19,12 +19,8 @@ def test_cpp_host_api_builds():
 
 
 def _write_scene(path, sc, W, H, spp, depth, cull, seed):
-    mats = np.zeros((len(sc.materials), 4), np.float32)
-    for i, m in enumerate(sc.materials):
-        mats[i, 0] = np.array([{"lambert": 0, "mirror": 1, "metal": 2}[m["kind"]]], np.int32).view(np.float32)[0]
-        mats[i, 1:] = m["albedo"]
-    lights = np.array([list(l["corner"]) + list(l["edge_u"]) + list(l["edge_v"]) + list(l["emission"])
-                       for l in sc.lights], np.float32)
+    mats = D.plugins.material_table(sc.materials)[: len(sc.materials)]     # dr_material rows (48 B)
+    lights = D.plugins.light_table(sc.lights)[: len(sc.lights)]           # dr_light rows (64 B)
     hdr = np.array([len(sc.tris), len(sc.materials), len(sc.lights), W, H, spp[0], spp[1], depth, int(cull),
                     (16 << 8) | 12], np.int32)
     with open(path, "wb") as f:

@@ -1,0 +1,311 @@
+"""The oracle's reference plugins pinned by closed forms, special cases and invariants.
+
+The oracle (oracle/dr_render_ref.inc) restates DumbBasedShader, SimpleStochasticShader,
+FresnelShader, DefaultShaderGeneric, the three reference lights and the reference
+estimator with photon transparency (DumbRenderer.cu:256-443).  The reference ships no
+golden vectors for any of them, so each test fixes a case whose value follows from the
+mathematics alone: normal incidence, mirror configurations, pass-through photons,
+sampling invariants, and whole renders of flat scenes whose pixels have closed forms.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import dumbray_amd as D
+from dumbray_amd import plugins as PL
+from dumbray_amd import scenes as S
+
+PI_REF = np.float32(3.14159265359)
+
+# a triangle in the plane y = 0 whose Triangle::normal (c - a) x (b - a) is +y
+TRI_UP = np.array([-1.0, 0.0, -1.0, 2.0, 0.0, -1.0, -1.0, 0.0, 2.0], np.float32)
+P0 = (0.0, 0.0, 0.0)
+WHITE = (1.0, 1.0, 1.0)
+
+
+def test_triangle_normal_convention():
+    a, b, c = TRI_UP.reshape(3, 3)
+    assert np.allclose(np.cross(c - a, b - a) / np.linalg.norm(np.cross(c - a, b - a)), (0, 1, 0))
+
+
+# ---------------------------------------------------------------- pow and the sphere sampler
+def test_pow_against_libm(oracle):
+    xs = np.concatenate([np.geomspace(1e-6, 4.0, 97), [0.25, 0.5, 0.999, 1.001]])
+    for y in (0.5, 2.0, 3.0, 8.0, 16.0, 128.0, 1.0 / 16.0, 1.0 / 4096.0, -1.5):
+        for x in xs:
+            e = float(x) ** y
+            if not (1e-30 < e < 1e30) or abs(y * math.log2(x)) > 60:
+                continue
+            r = oracle.pow_f(float(np.float32(x)), y)
+            ref = float(np.float32(x)) ** float(np.float32(y))
+            assert abs(r - ref) <= 4e-6 * abs(ref), (x, y, r, ref)
+
+
+def test_pow_special_cases(oracle):
+    assert oracle.pow_f(0.0, 3.0) == 0.0
+    assert oracle.pow_f(0.0, 0.0) == 1.0
+    assert oracle.pow_f(5.0, 0.0) == 1.0
+    assert oracle.pow_f(1.0, float("nan")) == 1.0
+    assert math.isinf(oracle.pow_f(0.0, -2.0))
+    assert abs(oracle.pow_f(-2.0, 3.0) + 8.0) < 1e-5          # odd integer power keeps the sign
+    assert abs(oracle.pow_f(-0.5, 8.0) - 0.00390625) < 1e-9    # even integer power of a negative base
+    assert math.isnan(oracle.pow_f(-2.0, 0.5))
+    assert oracle.pow_f(1e-20, 4096.0) == 0.0                    # underflow
+
+
+def test_point_on_sphere_radius_and_moments(oracle):
+    g = (np.arange(64) + 0.5) / 64.0
+    pts = np.array([oracle.point_on_sphere(a, b, 2.5) for a in g for b in g], np.float64)
+    assert np.allclose(np.linalg.norm(pts, axis=1), 2.5, rtol=1e-6)
+    u = pts / 2.5
+    assert np.all(np.abs(u.mean(0)) < 2e-3)                          # centred
+    assert np.allclose((u * u).mean(0), 1.0 / 3.0, atol=2e-3)       # isotropic second moments
+
+
+# ---------------------------------------------------------------- constructors
+def test_constructors_follow_the_reference():
+    f32 = np.float32
+    m = PL.simple_stochastic_shader((0.5, 0.6, 0.7), diffuse=0.8, smoothness=0.6, shininess=4.0)
+    tot = f32(0.8) + f32(0.6)
+    assert m["p"][0] == float(f32(0.8) * (f32(1) / tot)) and m["p"][1] == float(f32(0.6) * (f32(1) / tot))
+    m = PL.default_shader(diffuse=0.3, smoothness=0.2)          # sum below 1: unchanged
+    assert m["p"][:2] == (float(f32(0.3)), float(f32(0.2)))
+    m = PL.dumb_based_shader((0.1, 0.2, 0.3), 32.0, (0.5, 0.5, 0.5), metal=1.7)   # metal clamped to 1
+    assert m["p"][1] == 1.0 and m["diffuse"] == (0.0, 0.0, 0.0)
+    m = PL.dumb_based_shader(diffuse=(0.5, 0.4, 0.2), metal=0.25)
+    assert m["diffuse"] == tuple(float(f32(c) * f32(0.75)) for c in (0.5, 0.4, 0.2))
+    gg = PL.glossy_gold()
+    assert gg["albedo"] == tuple(float(f32(c)) for c in (1.022, 0.782, 0.344)) and gg["p"][0] == 128.0
+    dl = PL.directional_light((1, 1, 1), (0.0, -3.0, 4.0), 100.0)
+    assert np.allclose(dl["edge_u"], (0.0, -0.6, 0.8), atol=1e-7)
+    sl = PL.soft_directional_light(distance=1024.0, softness=0.125)
+    assert sl["radius"] == 128.0 and np.allclose(sl["edge_u"], np.array([1, -1, 1]) / math.sqrt(3), atol=1e-7)
+
+
+# ---------------------------------------------------------------- getReflectedColor
+def test_dumb_based_direct_at_normal_incidence(oracle):
+    """Photon straight down, observer straight up: wh = n, fresnel = F, pow(1, s) = 1, G = 1,
+    so brdf = clamp(F (s + 2) / (8 pi)) and colour = clamp(brdf metal + diffuse) * photon."""
+    for spec, metal in ((8.0, 0.5), (128.0, 0.25), (2.0, 0.9)):
+        F, diff, pc = (0.3, 0.5, 0.9), (0.6, 0.4, 0.2), (2.0, 1.0, 0.5)
+        m = PL.dumb_based_shader(F, spec, diff, metal)
+        out = oracle.reflected_color(m, TRI_UP, None, P0, (0, -1, 0), pc, (0, 1, 0), 0)
+        mult = (spec + 2.0) / (2.0 * float(PI_REF)) / 4.0
+        for c in range(3):
+            brdf = min(max(F[c] * mult, 0.0), 1.0)
+            want = min(max(brdf * metal + m["diffuse"][c], 0.0), 1.0) * pc[c]
+            assert abs(out[c] - want) <= 1e-5 * max(1.0, want), (spec, metal, c, out, want)
+
+
+def test_dumb_based_indirect_cases(oracle):
+    m = PL.dumb_based_shader((0.3, 0.5, 0.9), 16.0, (0.6, 0.4, 0.2), 0.5)
+    pc = (0.7, 0.8, 0.9)
+    # a photon arriving from behind the geometric normal passes straight through
+    assert np.array_equal(oracle.reflected_color(m, TRI_UP, None, P0, (0, 1, 0), pc, (0, 1, 0), 1, 2),
+                          np.float32(pc))
+    # observer behind the interpolated normal: black
+    assert np.all(oracle.reflected_color(m, TRI_UP, None, P0, (0, -1, 0), pc, (0, -1, 0), 0) == 0)
+    # diffuse sample (type 2): diffuse * cos(theta) / (1 - metal) * photon
+    th = 0.6
+    pd = (-math.sin(th), -math.cos(th), 0.0)
+    out = oracle.reflected_color(m, TRI_UP, None, P0, pd, pc, (0, 1, 0), 1, 2)
+    for c in range(3):
+        want = m["diffuse"][c] * math.cos(th) / (1.0 - 0.5) * pc[c]
+        assert abs(out[c] - want) < 2e-6, (out, want)
+    # specular sample (type 1) seen along the reversed photon: wh = wi, fresnel = F, no clamp reached
+    out = oracle.reflected_color(m, TRI_UP, None, P0, pd, pc, (math.sin(th), math.cos(th), 0.0), 1, 1)
+    assert np.allclose(out, np.array((0.3, 0.5, 0.9)) * pc, rtol=1e-6)
+
+
+def test_simple_stochastic_and_default_shader(oracle):
+    m = PL.simple_stochastic_shader((0.5, 0.25, 1.0), diffuse=0.3, smoothness=0.6, shininess=4.0)
+    d = PL.default_shader((0.5, 0.25, 1.0), diffuse=0.3, smoothness=0.6, shine=4.0)
+    pc = (2.0, 2.0, 2.0)
+    # photon straight down, observer along the mirror direction (up): both cosines are 1
+    out = oracle.reflected_color(m, TRI_UP, None, P0, (0, -1, 0), pc, (0, 1, 0), 0)
+    assert np.allclose(out, np.array((0.5, 0.25, 1.0)) * 2.0 * (0.3 + 0.6), rtol=1e-6)
+    # observer perpendicular to the mirror direction: only the diffuse term
+    out = oracle.reflected_color(m, TRI_UP, None, P0, (0, -1, 0), pc, (1, 0, 0), 0)
+    assert np.allclose(out, np.array((0.5, 0.25, 1.0)) * 2.0 * 0.3, rtol=1e-6)
+    # oblique photon at 45 degrees, observer at angle a from the mirror direction: pow(cos a, shine)
+    a = 0.4
+    r = np.array([math.sqrt(0.5), math.sqrt(0.5), 0.0])
+    ob = (math.cos(a) * r + math.sin(a) * np.array([-math.sqrt(0.5), math.sqrt(0.5), 0.0]))
+    out = oracle.reflected_color(m, TRI_UP, None, P0, (r[0], -r[1], 0.0), pc, ob, 1)
+    want = np.array((0.5, 0.25, 1.0)) * 2.0 * (0.3 * math.sqrt(0.5) + 0.6 * math.cos(a) ** 4)
+    assert np.allclose(out, want, rtol=1e-5)
+    # DefaultShaderGeneric::getReflectedColor is the same formula (the observer point at p + ob)
+    out_d = oracle.reflected_color(d, TRI_UP, None, P0, (r[0], -r[1], 0.0), pc, ob, 1)
+    assert np.allclose(out_d, want, rtol=1e-5)
+
+
+def test_fresnel_shader(oracle):
+    m = PL.fresnel_shader((0.2, 0.4, 0.6))
+    pc = (1.0, 2.0, 3.0)
+    assert np.allclose(oracle.reflected_color(m, TRI_UP, None, P0, (0, -1, 0), pc, (0, 1, 0), 0),
+                       np.array((0.2, 0.4, 0.6)) * pc, rtol=1e-6)       # normal incidence: F
+    assert np.allclose(oracle.reflected_color(m, TRI_UP, None, P0, (-1, 0, 0), pc, (0, 1, 0), 0), pc,
+                       rtol=1e-6)                                         # grazing: 1
+
+
+def test_vertex_normals_are_interpolated(oracle):
+    """nI = massCenter of the vertex normals at the hit point: a photon along the (tilted)
+    normal of the vertex where the hit point sits sees Fresnel = F."""
+    m = PL.fresnel_shader((0.25, 0.25, 0.25))
+    n_tilt = np.array([0.8, 0.6, 0.0], np.float32)
+    normals = np.concatenate([n_tilt, [0, 1, 0], [0, 1, 0]]).astype(np.float32)
+    pa = TRI_UP[:3]
+    out = oracle.reflected_color(m, TRI_UP, normals, pa, -n_tilt, WHITE, (0, 1, 0), 0)
+    assert np.allclose(out, 0.25, atol=1e-5)
+    out_flat = oracle.reflected_color(m, TRI_UP, None, pa, -n_tilt, WHITE, (0, 1, 0), 0)
+    assert not np.allclose(out_flat, 0.25, atol=1e-3)
+
+
+# ---------------------------------------------------------------- requestIndirectSamples
+def _many_samples(oracle, m, d, n=400):
+    out = []
+    for pix in range(n):
+        r = oracle.indirect_sample(m, TRI_UP, None, P0, d, pix=pix, samp=3, layer=1)
+        if r is not None:
+            out.append(r)
+    return out
+
+
+def test_dumb_based_samples(oracle):
+    d = (0.3, -1.0, 0.2)
+    diffuse = _many_samples(oracle, PL.dumb_based_shader(metal=0.0), d)
+    assert len(diffuse) == 400 and all(t == 2 for _, t in diffuse)
+    v = np.array([s for s, _ in diffuse])
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6) and np.all(v[:, 1] >= 0)
+    assert abs(v[:, 1].mean() - 0.5) < 0.04                     # uniform hemisphere: E[cos] = 1/2
+    spec = _many_samples(oracle, PL.dumb_based_shader(specular=4096.0, metal=1.0), d)
+    assert len(spec) == 400 and all(t == 1 for _, t in spec)
+    dn = np.array(d) / np.linalg.norm(d)
+    mirror = dn - 2 * dn.dot((0, 1, 0)) * np.array([0, 1, 0])
+    v = np.array([s for s, _ in spec])
+    ang = np.arccos(np.clip(v @ mirror / np.linalg.norm(v, axis=1), -1, 1))
+    assert np.all(ang < 0.1) and np.all(v[:, 1] >= 0)          # a tight lobe about the mirror direction
+    # a ray that arrives from behind the interpolated normal requests no specular sample
+    assert all(oracle.indirect_sample(PL.dumb_based_shader(metal=1.0), TRI_UP, None, P0, (0, 1, 0), pix=k) is None
+               for k in range(20))
+
+
+def test_simple_stochastic_default_fresnel_samples(oracle):
+    d = (0.5, -1.0, 0.0)
+    mirror = np.array([0.5, 1.0, 0.0])
+    gl = _many_samples(oracle, PL.simple_stochastic_shader(diffuse=0.0, smoothness=1.0), d, 20)
+    for s, t in gl:
+        assert t == 0 and np.allclose(s, mirror / np.linalg.norm(mirror), atol=1e-6)
+    df = _many_samples(oracle, PL.simple_stochastic_shader(diffuse=1.0, smoothness=0.0), d)
+    v = np.array([s for s, _ in df])
+    assert np.all(v[:, 1] >= 0) and abs(v[:, 1].mean() - 0.5) < 0.04
+    ds = oracle.indirect_sample(PL.default_shader(), TRI_UP, None, P0, d)
+    assert ds is not None and np.allclose(ds[0], mirror, atol=1e-6)       # Vector3::reflection, not normalized
+    assert oracle.indirect_sample(PL.fresnel_shader(), TRI_UP, None, P0, d) is None
+
+
+# ---------------------------------------------------------------- lights
+def test_light_photons(oracle):
+    p = (1.0, 2.0, 3.0)
+    o, d, c = oracle.light_photon(PL.directional_light((1, 2, 3), (0, -1, 0), 100.0), p)
+    assert np.allclose(o, (1.0, 102.0, 3.0)) and np.allclose(d, (0, -1, 0)) and np.allclose(c, (1, 2, 3))
+    base = np.array(p) - np.array(PL.soft_directional_light()["edge_u"]) * 1024.0
+    for pix in range(50):
+        o, d, c = oracle.light_photon(PL.soft_directional_light(), p, pix=pix)
+        assert np.linalg.norm(o - base) <= 128.0 * (1 + 1e-6)
+        assert np.allclose(d, np.array(p) - o, atol=1e-3)
+    for pix in range(50):
+        o, d, c = oracle.light_photon(PL.spherical_light((4, 4, 4), (0, 5, 0), 0.5), p, pix=pix)
+        assert np.linalg.norm(o - np.array([0, 5, 0])) <= 0.5 * (1 + 1e-6)
+        delta = np.array(p) - o
+        assert np.allclose(d, delta / np.linalg.norm(delta), atol=1e-6)
+        assert np.allclose(c, 4.0 / delta.dot(delta), rtol=1e-5)
+
+
+# ---------------------------------------------------------------- whole renders with closed forms
+def _floor(y=0.0, half=20.0):
+    # two triangles whose Triangle::normal is +y
+    a, b, c, d = (-half, y, -half), (-half, y, half), (half, y, half), (half, y, -half)
+    return np.array([a + d + b, b + d + c], np.float32)
+
+
+def _camera_down(height):
+    cam = S.Camera.look_at((0.0, height, 0.0), (0.0, 0.0, 0.001), fov_deg=60.0)
+    return cam
+
+
+def _render_ref(oracle, tris, mat_id, materials, lights, cam, W, H, **kw):
+    t = oracle.Octree(tris, 16, 12)
+    R = oracle.Renderer(t, mat_id, materials, lights, cam.as_array(), W, H, lens=1, **kw)
+    return R.render_ref(nthreads=4)
+
+
+def _camera_dirs(cam, W, H):
+    """Unit camera ray directions of the perspective lens at the pixel centres (1 spp)."""
+    c = cam.as_array().astype(np.float64)
+    P, R, U, F, f = c[0:3], c[3:6], c[6:9], c[9:12], c[12]
+    ys, xs = np.mgrid[0:H, 0:W]
+    sx = (xs - W / 2.0) / H + 0.5 / H
+    sy = (H / 2.0 - ys) / H + 0.5 / H
+    v = np.stack([sx, sy, np.full_like(sx, f)], -1)
+    v /= np.linalg.norm(v, axis=-1, keepdims=True)
+    return v[..., :1] * R + v[..., 1:2] * U + v[..., 2:3] * F
+
+
+def test_directional_light_on_dumb_based_floor_closed_form(oracle):
+    """One layer, light straight down: every pixel is clamp(brdf(metal) + diffuse) * E with
+    the observer = the reversed camera ray; computed per pixel in float64."""
+    W, H = 24, 16
+    spec, metal, F, diff, E = 8.0, 0.3, (0.9, 0.6, 0.3), (0.5, 0.5, 0.5), (1.5, 1.0, 0.5)
+    m = PL.dumb_based_shader(F, spec, diff, metal)
+    img = _render_ref(oracle, _floor(), [0, 0], [m], [PL.directional_light(E, (0, -1, 0), 50.0)], _camera_down(3.0),
+                      W, H, max_bounces=0)
+    dirs = _camera_dirs(_camera_down(3.0), W, H)
+    n = np.array([0.0, 1.0, 0.0])
+    wi = np.array([0.0, 1.0, 0.0])
+    w0 = -dirs
+    wh = w0 + wi
+    wh /= np.linalg.norm(wh, axis=-1, keepdims=True)
+    val = 1.0 - (wh @ wi)
+    fres = np.array(F) + (1.0 - np.array(F)) * (val ** 5)[..., None]
+    nwh, nw0 = wh @ n, w0 @ n
+    dwh = (spec + 2.0) / (2.0 * float(PI_REF)) * nwh ** spec
+    g = np.minimum(1.0, np.minimum(2 * nwh * nw0 / (w0 * wh).sum(-1), 2 * nwh * 1.0 / (w0 * wh).sum(-1)))
+    mult = dwh * g / (4.0 * nw0 * 1.0)
+    brdf = np.clip(fres * mult[..., None], 0, 1)
+    want = np.clip(brdf * metal + np.array(m["diffuse"]), 0, 1) * np.array(E)
+    assert np.allclose(img, want, rtol=2e-5, atol=1e-6), np.abs(img - want).max()
+
+
+def test_photon_passes_through_a_simple_stochastic_blocker(oracle):
+    """A SimpleStochasticShader sheet above the floor: the direct photon reaches the floor
+    with colour E * albedo * diffuse (the sheet's getReflectedColor toward the photon
+    direction), so the matte floor shows diffuse_floor * that colour (DumbRenderer.cu:396-413)."""
+    W, H = 16, 12
+    floor = PL.dumb_based_shader((0.0, 0.0, 0.0), 8.0, (0.8, 0.6, 0.4), 0.0)
+    sheet = PL.simple_stochastic_shader((0.5, 0.7, 0.9), diffuse=0.6, smoothness=0.0, shininess=2.0)
+    tris = np.concatenate([_floor(0.0), _floor(5.0)])
+    E = (2.0, 2.0, 2.0)
+    img = _render_ref(oracle, tris, [0, 0, 1, 1], [floor, sheet], [PL.directional_light(E, (0, -1, 0), 50.0)],
+                      _camera_down(3.0), W, H, max_bounces=0, ignore_backfaces=False)
+    want = np.array([0.8, 0.6, 0.4]) * np.array(E) * np.array([0.5, 0.7, 0.9]) * 0.6
+    close = np.all(np.isclose(img, want, rtol=1e-5), axis=-1)
+    assert close.mean() > 0.95, img.reshape(-1, 3)[~close.reshape(-1)][:4]
+    # an opaque (benchmark Lambert) sheet blocks the light completely
+    img2 = _render_ref(oracle, tris, [0, 0, 1, 1], [floor, PL.lambert((0.5, 0.5, 0.5))],
+                       [PL.directional_light(E, (0, -1, 0), 50.0)], _camera_down(3.0), W, H, max_bounces=0,
+                       ignore_backfaces=False)
+    assert np.all(img2 == 0)
+
+
+def test_reference_estimator_equals_benchmark_estimator_on_benchmark_plugins(oracle):
+    """With only the benchmark plugins the reference estimator renders what the benchmark
+    estimator renders (same samples; sums in a different order)."""
+    sc = S.spheres_scene(4, seed=2)
+    t = oracle.Octree(sc.tris, 16, 12)
+    R = oracle.Renderer(t, sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(), 40, 30, (2, 2), 3)
+    a = R.render(nthreads=4)
+    b = R.render_ref(nthreads=4)
+    assert a.max() > 0
+    assert np.allclose(a, b, rtol=2e-5, atol=1e-6)

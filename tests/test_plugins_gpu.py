@@ -1,0 +1,151 @@
+"""GPU parity of the reference plugins (DR_PLUGINS_REFERENCE) and of the lens choice with
+the oracle on identical seeded scenes.
+
+As in the benchmark estimator, rays, hits, samples, photons and their transmissions are
+computed with the same fp32 expressions on both sides; only the accumulation order of
+the radiance differs (throughput forward on the GPU, layer colours bottom-up in the
+oracle).  Tolerance per pixel: |gpu - oracle| <= 1e-4 * |oracle| + 1e-6."""
+import numpy as np
+import pytest
+
+import dumbray_amd as D
+from dumbray_amd import plugins as PL
+from dumbray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+RTOL, ATOL = 1e-4, 1e-6
+
+
+def _reference_scene(seed, shader_cycle, lights, n_spheres=6):
+    sc = S.spheres_scene(n_spheres, seed=seed)
+    mats = []
+    for i, m in enumerate(sc.materials):
+        make = shader_cycle[i % len(shader_cycle)]
+        mats.append(make(m["albedo"]))
+    sc.materials = mats
+    sc.lights = lights
+    return sc
+
+
+SHADERS = {
+    "dumb_based": lambda a: PL.dumb_based_shader((0.9, 0.7, 0.4), 32.0, a, 0.4),
+    "simple_stochastic": lambda a: PL.simple_stochastic_shader(a, 0.6, 0.5, 8.0),
+    "fresnel": lambda a: PL.fresnel_shader(a),
+    "default": lambda a: PL.default_shader(a, 0.7, 0.2, 8.0),
+    "mixed": None,
+}
+MIXED = [lambda a: PL.dumb_based_shader((0.9, 0.7, 0.4), 16.0, a, 0.5), lambda a: PL.lambert(a),
+         lambda a: PL.simple_stochastic_shader(a, 0.3, 0.6, 4.0), lambda a: PL.default_shader(a),
+         lambda a: PL.matte_finish(), lambda a: PL.metal((1.0, 0.78, 0.34))]
+LIGHTS = [PL.directional_light((1.5, 1.4, 1.2), (0.3, -1.0, 0.2), 40.0),
+          PL.soft_directional_light((1.0, 1.0, 1.0), (-0.2, -1.0, 0.4), 40.0, 0.05),
+          PL.spherical_light((40.0, 40.0, 40.0), (1.0, 6.0, -1.0), 0.7),
+          PL.area_light((-0.5, 5.0, -0.5), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (10.0, 10.0, 10.0))]
+
+
+def _pair(oracle, sc, W, H, spp, depth, normals=None, lens="stochastic", **kw):
+    tree = D.Octree(16, 12)
+    tree.push(sc.tris)
+    tree.build()
+    g = D.DumbRenderer(tree.upload(), sc.mat_id, sc.materials, sc.lights, sc.camera, W, H, spp, depth,
+                       normals=normals, lens=lens, **kw)
+    o = oracle.Renderer(oracle.Octree(sc.tris, 16, 12), sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(),
+                        W, H, spp, depth, lens=D.render.LENSES[lens], normals=normals, **kw)
+    return g, o
+
+
+def _gpu(g, iteration=1):
+    import torch
+    f = g.render(None, iteration)
+    torch.cuda.synchronize()
+    return f.to_image()[..., :3].cpu().numpy()
+
+
+def _close(a, b):
+    err = np.abs(a - b)
+    bad = err > RTOL * np.abs(b) + ATOL
+    assert not bad.any(), (int(bad.sum()), float(err.max()), np.argwhere(bad)[:5].tolist())
+
+
+@pytest.mark.parametrize("shader", list(SHADERS))
+def test_reference_shaders_match_oracle(dev, oracle, shader):
+    cycle = MIXED if shader == "mixed" else [SHADERS[shader]]
+    sc = _reference_scene(5, cycle, LIGHTS)
+    print(f"[case] reference shader {shader} 48x36 spp 2x2 depth 3", flush=True)
+    g, o = _pair(oracle, sc, 48, 36, (2, 2), 3)
+    assert g.plugins == PL.PLUGINS_REFERENCE
+    img = _gpu(g)
+    ref = o.render_ref(nthreads=8)
+    assert ref.max() > 0
+    _close(img, ref)
+
+
+@pytest.mark.parametrize("light", range(len(LIGHTS)))
+def test_reference_lights_match_oracle(dev, oracle, light):
+    sc = _reference_scene(7, MIXED, [LIGHTS[light]])
+    print(f"[case] reference light {LIGHTS[light]['kind']} 40x30 spp 2x1 depth 2", flush=True)
+    g, o = _pair(oracle, sc, 40, 30, (2, 1), 2, ignore_backfaces=False)
+    img = _gpu(g)
+    ref = o.render_ref(nthreads=8)
+    assert ref.max() > 0
+    _close(img, ref)
+
+
+def test_transparent_blockers_and_vertex_normals_match_oracle(dev, oracle):
+    """SimpleStochastic spheres pass light on (photon transparency), smooth vertex normals
+    (unit vector from each sphere's centre) shade every reference shader."""
+    sc = _reference_scene(11, [lambda a: PL.simple_stochastic_shader(a, 0.7, 0.2, 2.0), MIXED[0], MIXED[3]],
+                          [LIGHTS[0], LIGHTS[2]], n_spheres=5)
+    v = sc.tris.reshape(-1, 3, 3).astype(np.float64)
+    cen = v.mean(1, keepdims=True)
+    nrm = (v - cen) / np.maximum(np.linalg.norm(v - cen, axis=-1, keepdims=True), 1e-9)
+    print("[case] transparency + vertex normals 40x30 spp 2x2 depth 3", flush=True)
+    g, o = _pair(oracle, sc, 40, 30, (2, 2), 3, normals=nrm.astype(np.float32).reshape(-1, 9),
+                 ignore_backfaces=False)
+    img = _gpu(g)
+    ref = o.render_ref(nthreads=8)
+    assert ref.max() > 0
+    _close(img, ref)
+
+
+@pytest.mark.parametrize("plugins", ["benchmark", "reference"])
+def test_perspective_lens_matches_oracle(dev, oracle, plugins):
+    sc = S.spheres_scene(6, seed=4)
+    if plugins == "reference":
+        sc.materials = [PL.dumb_based_shader((0.5, 0.5, 0.5), 8.0, m["albedo"], 0.2) for m in sc.materials]
+    print(f"[case] perspective lens ({plugins} plugins) 40x30 spp 2x2 depth 3", flush=True)
+    g, o = _pair(oracle, sc, 40, 30, (2, 2), 3, lens="perspective")
+    img = _gpu(g)
+    ref = o.render(nthreads=8) if plugins == "benchmark" else o.render_ref(nthreads=8)
+    assert ref.max() > 0
+    _close(img, ref)
+
+
+def test_reference_estimator_on_benchmark_plugins_matches_benchmark_kernel(dev):
+    """Forcing the reference estimator on a benchmark-plugin scene renders the benchmark
+    kernel's image (the same paths; only the order of one sum differs)."""
+    sc = S.icosphere_scene(level=3, seed=3)
+    tree = D.Octree(16, 12)
+    tree.push(sc.tris)
+    tree.build()
+    g = D.DumbRenderer(tree.upload(), sc.mat_id, sc.materials, sc.lights, sc.camera, 48, 40, (2, 2), 6,
+                       rr_start=3, rr_survive=0.75)
+    a = _gpu(g)
+    g.plugins = PL.PLUGINS_REFERENCE
+    b = _gpu(g)
+    assert a.max() > 0
+    _close(b, a)
+
+
+def test_adjoint_and_batches_reject_reference_plugins(dev):
+    import torch
+    sc = _reference_scene(5, MIXED, LIGHTS[:1], n_spheres=2)
+    tree = D.Octree(16, 12)
+    tree.push(sc.tris)
+    tree.build()
+    g = D.DumbRenderer(tree.upload(), sc.mat_id, sc.materials, sc.lights, sc.camera, 16, 16, (1, 1), 1)
+    f = g.render()
+    with pytest.raises(D.DumbRayError):
+        g.backward(torch.zeros_like(f.data))
+    with pytest.raises(D.DumbRayError):
+        D.RenderBatch([(g, f, 1, 0)], tuning={"tile_blocks": 1, "groups_per_cu": 1, "refill_min": 1})
