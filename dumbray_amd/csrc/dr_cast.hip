@@ -3,7 +3,8 @@
 // A persistent grid of 256-lane blocks; each lane carries one ray and its own
 // DFS stack in LDS.  The loop advances every traversing lane by one bounded step
 // per iteration and batches the rarer, divergent work (leaf tests, taking new
-// rays) until enough lanes need it.
+// rays) until enough lanes need it; a leaf batch spreads the records of all
+// pending leaves over the whole wave (walk_leaf_wave).
 #include <algorithm>
 #include "dr_common.h"
 
@@ -43,7 +44,11 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
                                                       float *__restrict__ out_t, float *__restrict__ out_p,
                                                       int *__restrict__ stats, unsigned *__restrict__ work) {
   extern __shared__ uint32_t stk_base[];
+  __shared__ unsigned long long best_all[kCastBlock];   // per lane: best (t, record) key of its leaf
+  __shared__ int ins_all[kStats ? kCastBlock : 1];      // per lane: inside tests of its leaf (measurement)
   uint32_t *stk = stk_base + threadIdx.x;
+  unsigned long long *best = best_all + (threadIdx.x & ~63u);
+  int *ins = kStats ? ins_all + (threadIdx.x & ~63u) : nullptr;
   constexpr int S = kCastBlock;
 
   const int lane = (int)__lane_id();
@@ -111,13 +116,13 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
       walk_step(T, w, stk, S, cc);
       if (w.state == kWalkDone) finish();
     }
-    // ---- castInLeaf for the pending leaves
-    const uint64_t pm = __ballot(r >= 0 && w.state == kWalkLeaf);
+
+    // ---- castInLeaf for the pending leaves, the whole wave sharing their records
+    const bool pend = r >= 0 && w.state == kWalkLeaf;
+    const uint64_t pm = __ballot(pend);
     if (pm && (__popcll(pm) >= kLeafMin || __ballot(r >= 0 && w.state == kWalkTrace) == 0)) {
-      if (r >= 0 && w.state == kWalkLeaf) {
-        walk_leaf(T, w, clip != 0, cc);
-        if (w.state == kWalkDone) finish();
-      }
+      walk_leaf_wave(T, w, pend, clip != 0, best, ins, cc);
+      if (pend && w.state == kWalkDone) finish();
     }
   }
   // ---- the last wave out resets the counters

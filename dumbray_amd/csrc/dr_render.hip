@@ -318,6 +318,7 @@ struct BlockLds {
   float *samp_col;      // kBlockPixels * kRoundSamples * 3 sample colours
   float *acc;           // kBlockPixels * 3 pixel sums
   int *next_job;
+  unsigned long long *best;   // this wave's leaf-test slots (walk_leaf_wave)
 };
 
 // Renders framebuffer block `block` of one request (all lanes of the workgroup call it).
@@ -382,10 +383,10 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
       // ---- one traversal step of every ray in flight
       if (job >= 0 && w.state == kWalkTrace) walk_step(T, w, lds.stk, kRB, nullptr);
       // ---- leaf tests, batched
-      const uint64_t pm = __ballot(job >= 0 && w.state == kWalkLeaf);
-      if (pm && (__popcll(pm) >= kLeafMin || __ballot(job >= 0 && w.state == kWalkTrace) == 0)) {
-        if (job >= 0 && w.state == kWalkLeaf) walk_leaf(T, w, clip, nullptr);
-      }
+      const bool pend = job >= 0 && w.state == kWalkLeaf;
+      const uint64_t pm = __ballot(pend);
+      if (pm && (__popcll(pm) >= kLeafMin || __ballot(job >= 0 && w.state == kWalkTrace) == 0))
+        walk_leaf_wave(T, w, pend, clip, lds.best, nullptr, nullptr);
       // ---- path updates of the finished casts, batched
       const uint64_t dm = __ballot(job >= 0 && w.state == kWalkDone);
       if (dm && (__popcll(dm) >= kShadeMin || __ballot(job >= 0 && w.state != kWalkDone) == 0)) {
@@ -428,7 +429,9 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];
   __shared__ int next_job;
-  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job};
+  __shared__ unsigned long long best[kRB];
+  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
+                      best + (threadIdx.x & ~63u)};
   render_block<PathT>(T, tb, P, frame, start_block + blockIdx.x, L, 1);
 }
 
@@ -452,6 +455,8 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
                                                                  int maxv) {
   extern __shared__ uint32_t lds[];
   __shared__ int next_job;
+  __shared__ unsigned long long best_all[kRB];
+  unsigned long long *best = best_all + (threadIdx.x & ~63u);
   uint32_t *stk = lds + threadIdx.x;
   float *gacc = (float *)(lds + levels * kStackWordsPerLevel * kRB);   // nmat*3 albedo, then nlights*3 emission
   float *vtx = vtx_all + (size_t)blockIdx.x * maxv * kVtxWords * kRB;
@@ -538,10 +543,10 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
         continue;
       }
       if (st.job >= 0 && w.state == kWalkTrace) walk_step(T, w, stk, kRB, nullptr);
-      const uint64_t pm = __ballot(st.job >= 0 && w.state == kWalkLeaf);
-      if (pm && (__popcll(pm) >= kLeafMin || __ballot(st.job >= 0 && w.state == kWalkTrace) == 0)) {
-        if (st.job >= 0 && w.state == kWalkLeaf) walk_leaf(T, w, clip, nullptr);
-      }
+      const bool pend = st.job >= 0 && w.state == kWalkLeaf;
+      const uint64_t pm = __ballot(pend);
+      if (pm && (__popcll(pm) >= kLeafMin || __ballot(st.job >= 0 && w.state == kWalkTrace) == 0))
+        walk_leaf_wave(T, w, pend, clip, best, nullptr, nullptr);
       const uint64_t dm = __ballot(st.job >= 0 && w.state == kWalkDone);
       if (dm && (__popcll(dm) >= kShadeMin || __ballot(st.job >= 0 && w.state != kWalkDone) == 0)) {
         if (st.job >= 0 && w.state == kWalkDone) {
@@ -604,7 +609,9 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
   __shared__ float acc[kBlockPixels * 3];
   __shared__ int next_job;
   __shared__ int item;
-  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job};
+  __shared__ unsigned long long best[kRB];
+  const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
+                      best + (threadIdx.x & ~63u)};
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(counter, 1);
     __syncthreads();

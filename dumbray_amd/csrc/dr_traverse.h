@@ -14,6 +14,7 @@
 #include "dr_geom.h"
 #include "dr_octree_build.h"
 
+
 namespace dr {
 
 struct DevTree {
@@ -42,6 +43,7 @@ struct CastCounters {
 struct TRay {
   V3 o, d, inv;
   int canon;     // canonicalOrder (Octree.impl.cuh:181-183)
+  V3 hi, lo;     // prune thresholds: a child is behind if max.k < hi.k or min.k > lo.k (see set_prune)
 };
 __device__ __forceinline__ TRay make_tray(V3 o, V3 d) {
   TRay r;
@@ -53,6 +55,23 @@ __device__ __forceinline__ TRay make_tray(V3 o, V3 d) {
   if (d.x < 0) c += 4;
   r.canon = c;
   return r;
+}
+
+// v rounded down (up) by at least one ulp of v: below (above) every real number that
+// rounds to v.
+__device__ __forceinline__ float below(float v) { return v - (__builtin_fabsf(v) * 2.4e-7f + 1e-37f); }
+__device__ __forceinline__ float above(float v) { return v + (__builtin_fabsf(v) * 2.4e-7f + 1e-37f); }
+// Prune thresholds of the forward-octant test for margin M.  On an axis with d.k >= 0
+// a child whose box max satisfies max.k < hi.k = below(o.k - M) has max.k + M < o.k in
+// exact arithmetic, so every non-empty leaf below it (within the box widened by M) lies
+// behind o along k; symmetrically lo.k = above(o.k + M) for d.k <= 0.  Axes that do not
+// apply get -inf / +inf.
+__device__ __forceinline__ void set_prune(TRay &r, float M) {
+  const float inf = __builtin_inff();
+  r.hi = v3(r.d.x >= 0 ? below(r.o.x - M) : -inf, r.d.y >= 0 ? below(r.o.y - M) : -inf,
+            r.d.z >= 0 ? below(r.o.z - M) : -inf);
+  r.lo = v3(r.d.x <= 0 ? above(r.o.x + M) : inf, r.d.y <= 0 ? above(r.o.y + M) : inf,
+            r.d.z <= 0 ? above(r.o.z + M) : inf);
 }
 
 // Triangle::containsVertex minus the degenerate test (degenerate triangles have no record)
@@ -131,9 +150,9 @@ constexpr int kStackWordsPerLevel = 2;
 
 // 8 child tests of group g -> stack entry.  A child is kept if it has a non-empty
 // leaf below it, its box widened by the prune margin meets the forward octant of
-// the ray, and the reference slab test on its box passes.
-__device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g, const TRay &r, float M,
-                                            uint32_t &lo, uint32_t &hi) {
+// the ray (the thresholds of set_prune), and the reference slab test on its box passes.
+__device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g, const TRay &r, uint32_t &lo,
+                                            uint32_t &hi) {
   const float4 *q = G + kGroupF4 * (size_t)g;
   uint32_t m = 0;
 #pragma unroll
@@ -145,9 +164,8 @@ __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g,
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       // forward octant of o: coordinate k of o + d*t (t >= 0) is >= o.k if d.k >= 0, <= o.k if d.k <= 0
-      const bool behind = (r.d.x >= 0 && a_ex[j] + M < r.o.x) || (r.d.x <= 0 && a_sx[j] - M > r.o.x) ||
-                          (r.d.y >= 0 && a_ey[j] + M < r.o.y) || (r.d.y <= 0 && a_sy[j] - M > r.o.y) ||
-                          (r.d.z >= 0 && a_ez[j] + M < r.o.z) || (r.d.z <= 0 && a_sz[j] - M > r.o.z);
+      const bool behind = (a_ex[j] < r.hi.x) || (a_sx[j] > r.lo.x) || (a_ey[j] < r.hi.y) || (a_sy[j] > r.lo.y) ||
+                          (a_ez[j] < r.hi.z) || (a_sz[j] > r.lo.z);
       const bool hit = !behind && slab_hit(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv);
       m |= hit ? (1u << (4 * h + j)) : 0u;
     }
@@ -162,43 +180,6 @@ __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g,
   hi = (head.y << 8) | ((head.x >> 8) & 0xffu);
 }
 
-// Pops the next child to visit.  Returns 1 with (g, k, link, cnt) set when a
-// non-empty leaf was reached, pushes interior children, returns 0 when the
-// stack is exhausted (miss).
-__device__ __forceinline__ int next_leaf(const float4 *__restrict__ G, float M, const TRay &r, uint32_t *stk,
-                                         int stride, int &sp, int &g_out, int &k_out, int &link_out, int &cnt_out,
-                                         CastCounters *cc = nullptr) {
-  const int *Gi = (const int *)G;
-  while (sp >= 0) {
-    uint32_t *e = stk + 2 * sp * stride;
-    const uint32_t f = e[0];
-    const uint32_t m = f & 0xffu;
-    if (m == 0) { --sp; continue; }
-    const int i = __builtin_ctz(m);
-    e[0] = f & ~(1u << i);
-    const int k = r.canon ^ i;
-    const uint32_t hi = e[stride];
-    const uint32_t im = hi & 0xffu;
-    if ((im >> k) & 1u) {
-      const int child = (int)(hi >> 8) + __builtin_popcount(im & ((1u << k) - 1u));
-      ++sp;
-      uint32_t lo2, hi2;
-      group_entry(G, child, r, M, lo2, hi2);
-      stk[2 * sp * stride] = lo2;
-      stk[(2 * sp + 1) * stride] = hi2;
-      if (cc) cc->expand++;
-    } else {
-      const int g = (int)(f >> 8);
-      const size_t base = kGroupWords * (size_t)g;
-      g_out = g; k_out = k;
-      link_out = Gi[base + kGroupLink + k];
-      cnt_out = Gi[base + kGroupCount + k];
-      return 1;
-    }
-  }
-  return 0;
-}
-
 // Initial traversal state.  Returns 0 = miss, 1 = stack ready, 2 = root is a leaf.
 __device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint32_t *stk, int stride, int &sp,
                                           CastCounters *cc = nullptr) {
@@ -206,7 +187,7 @@ __device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint3
   if (!slab_hit(T.rsx, T.rsy, T.rsz, T.rex, T.rey, T.rez, r.o, r.inv)) return 0;
   sp = 0;
   uint32_t lo, hi;
-  group_entry(T.groups, 0, r, T.margin, lo, hi);     // group 0 = the root's children
+  group_entry(T.groups, 0, r, lo, hi);     // group 0 = the root's children
   stk[0] = lo;
   stk[stride] = hi;
   if (cc) cc->expand++;
@@ -228,6 +209,7 @@ struct Walk {
 __device__ __forceinline__ void walk_begin(const DevTree &T, Walk &w, V3 o, V3 d, int exclude, uint32_t *stk,
                                            int stride, CastCounters *cc) {
   w.ray = make_tray(o, d);
+  set_prune(w.ray, T.margin);
   w.ex = exclude;
   w.hit = false;
   const int st = tree_begin(T, w.ray, stk, stride, w.sp, cc);
@@ -259,7 +241,7 @@ __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *s
   if ((im >> k) & 1u) {
     const int child = (int)(hi >> 8) + __builtin_popcount(im & ((1u << k) - 1u));
     uint32_t lo2, hi2;
-    group_entry(T.groups, child, w.ray, T.margin, lo2, hi2);
+    group_entry(T.groups, child, w.ray, lo2, hi2);
     ++w.sp;
     stk[2 * w.sp * stride] = lo2;
     stk[(2 * w.sp + 1) * stride] = hi2;
@@ -285,23 +267,96 @@ __device__ __forceinline__ void walk_leaf(const DevTree &T, Walk &w, bool clip, 
   w.state = (w.hit || w.lg < 0) ? kWalkDone : kWalkTrace;
 }
 
-// Octree::cast for one ray.  stk points at this lane's first stack word;
-// consecutive stack words are `stride` apart (LDS, [word][lane]), and the stack
-// has kStackWordsPerLevel words per tree level.
-__device__ __forceinline__ bool tree_cast(const DevTree &T, V3 o, V3 d, bool clip, int exclude, uint32_t *stk,
-                                          int stride, Hit &h) {
-  const TRay r = make_tray(o, d);
-  int sp = -1;
-  const int st = tree_begin(T, r, stk, stride, sp);
-  if (st == 0) return false;
-  if (st == 2) {
-    Box rb; rb.s = v3(T.rsx, T.rsy, T.rsz); rb.e = v3(T.rex, T.rey, T.rez);
-    return leaf_cast(T.recs, T.root_start, T.root_count, rb, o, d, clip, exclude, h);
+// castInLeaf for every pending lane of the wave at once, with all 64 lanes sharing
+// the records.  Must be called by every lane of the wave (wave-uniform control flow);
+// `pending` marks the lanes whose leaf is tested; `best` (and `ins`, counting the
+// records that reach the inside test, for measurement launches) hold one slot per
+// lane of the wave in LDS.
+//   * The records of the pending leaves are numbered 0..total-1 in lane order (wave
+//     prefix sum of the counts); lane j tests records j, j + 64, ...
+//   * A record's owner is the last lane whose first record number is <= the record's
+//     (binary search over the prefix sums, read with __shfl); its ray, exclusion and
+//     first record come from the owner the same way.
+//   * A record that passes Triangle::cast with t < FLT_MAX offers the 64-bit key
+//     (t bits << 32 | record's position in its leaf) to the owner's slot in `best`
+//     (LDS, one uint64 per lane) with an atomic min.  t > 0, so the keys order by t
+//     first and by position second: the owner gets the first record with the smallest
+//     t, which is what the sequential strict '<' scan keeps (Octree.impl.cuh:595-606).
+//   * The owner then applies the leaf-box acceptance.
+__device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool pending, bool clip,
+                                               unsigned long long *best, int *ins, CastCounters *cc) {
+  const int lane = (int)__lane_id();
+  const int cnt = pending ? w.lcnt : 0;
+  int incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
   }
-  int g, k, link, cnt;
-  while (next_leaf(T.groups, T.margin, r, stk, stride, sp, g, k, link, cnt))
-    if (leaf_cast(T.recs, link, cnt, child_box(T.groups, g, k), o, d, clip, exclude, h)) return true;
-  return false;
+  const int start = incl - cnt;
+  const int total = __shfl(incl, 63);
+  if (pending) __hip_atomic_store(&best[lane], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  if (cc && pending) __hip_atomic_store(&ins[lane], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  const int link = pending ? w.llink : 0;
+  for (int base = 0; base < total; base += 64) {
+    const int j = base + lane;
+    int owner = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1) {
+      const int s = __shfl(start, owner + step);
+      if (s <= j) owner += step;
+    }
+    const V3 o = v3(__shfl(w.ray.o.x, owner), __shfl(w.ray.o.y, owner), __shfl(w.ray.o.z, owner));
+    const V3 d = v3(__shfl(w.ray.d.x, owner), __shfl(w.ray.d.y, owner), __shfl(w.ray.d.z, owner));
+    const int pos = j - __shfl(start, owner);
+    const int k = __shfl(link, owner) + pos;
+    const int ex = __shfl(w.ex, owner);
+    if (j < total) {
+      const float4 *R = T.recs;
+      const float4 r0 = R[4 * k + 0], r1 = R[4 * k + 1];
+      const V3 a = v3(r0.x, r0.y, r0.z), n = v3(r0.w, r1.x, r1.y);
+      const float dp = dot(a - o, n);
+      const float dd = dot(d, n);
+      const float t = dp / dd;
+      if (!(clip && (dp < -kVecEps)) && !(dp * dd <= 0) && (t < FLT_MAX)) {
+        if (cc) __hip_atomic_fetch_add(&ins[owner], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const float4 r2 = R[4 * k + 2], r3 = R[4 * k + 3];
+        if (__float_as_int(r3.w) != ex) {
+          const V3 p = o + d * t;
+          const V3 b = v3(r1.z, r1.w, r2.x), c = v3(r2.y, r2.z, r2.w);
+          if (inside(p, a, b, c, r3.x, r3.y, r3.z)) {
+            const unsigned long long key =
+                ((unsigned long long)__float_as_uint(t) << 32) | (unsigned long long)(unsigned)pos;
+            __hip_atomic_fetch_min(&best[owner], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+          }
+        }
+      }
+    }
+  }
+  if (!pending) return;
+  if (cc) {
+    cc->leaves++;
+    cc->plane += cnt;
+    cc->inside += __hip_atomic_load(&ins[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
+  const unsigned long long key = __hip_atomic_load(&best[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  bool hit = false;
+  if (key != ~0ull) {
+    const float t = __uint_as_float((unsigned)(key >> 32));
+    const int k = w.llink + (int)(key & 0xffffffffu);
+    const V3 p = w.ray.o + w.ray.d * t;
+    Box lb;
+    if (w.lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
+    else lb = child_box(T.groups, w.lg, w.lk);
+    if (contains(lb, p)) {
+      hit = true;
+      w.h.id = __float_as_int(T.recs[4 * k + 3].w);
+      w.h.t = t;
+      w.h.p = p;
+    }
+  }
+  w.hit = hit;
+  w.state = (hit || w.lg < 0) ? kWalkDone : kWalkTrace;
 }
 
 }  // namespace dr
