@@ -228,6 +228,7 @@ def main():
                     help="only the timed incoherent casts (+ calibration copy): for rocprofv3 runs")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing and adjoint legs")
     ap.add_argument("--adam-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-requests leg")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -353,7 +354,7 @@ def main():
         render = {"path_trace_1080p": bench_path_trace(D, S, stream, args.cpu_seconds, not args.no_cpu),
                   "path_trace_256": bench_small(D, S, stream),
                   "adjoint_512": bench_adjoint(D, S, stream, args.adam_steps),
-                  "batch_64_requests": bench_batch(D, S, stream)}
+                  "batch_64_requests": None if args.no_batch else bench_batch(D, S, stream)}
 
     line = {
         "metric": "closest-hit Mrays/s & Mpix·spp/s at 1080p; adjoint/forward time ratio",

@@ -14,7 +14,6 @@
 #include "dr_geom.h"
 #include "dr_octree_build.h"
 
-
 namespace dr {
 
 struct DevTree {
@@ -81,56 +80,6 @@ __device__ __forceinline__ bool inside(V3 p, V3 a, V3 b, V3 c, float thr0, float
   const V3 ax = p - a, bx = p - b, cx = p - c;
   return (dot(ab, ax) / mag(ax) + kInsideTol >= thr0) && (dot(bc, bx) / mag(bx) + kInsideTol >= thr1) &&
          (dot(ca, cx) / mag(cx) + kInsideTol >= thr2);
-}
-
-// castInLeaf: closest accepted triangle among `count` records; accepted iff its hit
-// point lies inside the leaf box.  exclude = validateNotSameAsObject's object (or -1).
-// Records are read kLeafChunk at a time so their loads are in flight together;
-// the candidates of a chunk are then resolved in record order, which keeps the
-// reference's strict '<' first-wins rule.
-constexpr int kLeafChunk = 4;
-template <int C = kLeafChunk>
-__device__ __forceinline__ bool leaf_cast(const float4 *__restrict__ R, int start, int count, const Box &lb, V3 o,
-                                          V3 d, bool clip, int exclude, Hit &h, CastCounters *cc = nullptr) {
-  if (cc) { cc->leaves++; cc->plane += count; }
-  float best = FLT_MAX;
-  V3 bp = v3(0.f, 0.f, 0.f);
-  int bid = -1;
-  const int end = start + count;
-  for (int k0 = start; k0 < end; k0 += C) {
-    float4 r0[C], r1[C];
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-      const int k = min(k0 + j, end - 1);
-      r0[j] = R[4 * k + 0];
-      r1[j] = R[4 * k + 1];
-    }
-    float tj[C];
-    bool cand[C];
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-      const V3 a = v3(r0[j].x, r0[j].y, r0[j].z), n = v3(r0[j].w, r1[j].x, r1[j].y);
-      const float dp = dot(a - o, n);
-      const float dd = dot(d, n);
-      tj[j] = dp / dd;
-      cand[j] = (k0 + j < end) && !(clip && (dp < -kVecEps)) && !(dp * dd <= 0) && (tj[j] < best);
-    }
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-      if (!cand[j] || !(tj[j] < best)) continue;                 // could not replace the current best
-      if (cc) cc->inside++;
-      const int k = k0 + j;
-      const float4 r2 = R[4 * k + 2], r3 = R[4 * k + 3];
-      const int id = __float_as_int(r3.w);
-      if (id == exclude) continue;
-      const V3 p = o + d * tj[j];
-      const V3 a = v3(r0[j].x, r0[j].y, r0[j].z);
-      const V3 b = v3(r1[j].z, r1[j].w, r2.x), c = v3(r2.y, r2.z, r2.w);
-      if (inside(p, a, b, c, r3.x, r3.y, r3.z)) { best = tj[j]; bp = p; bid = id; }
-    }
-  }
-  if (best != FLT_MAX && contains(lb, bp)) { h.id = bid; h.t = best; h.p = bp; return true; }
-  return false;
 }
 
 // Box of child k of group g (the leaf-acceptance box).
@@ -255,16 +204,6 @@ __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *s
     w.lcnt = Gi[base + kGroupCount + k];
     w.state = kWalkLeaf;
   }
-}
-
-// castInLeaf of the pending leaf; an accepted hit (or the root leaf) ends the query.
-template <int C = kLeafChunk>
-__device__ __forceinline__ void walk_leaf(const DevTree &T, Walk &w, bool clip, CastCounters *cc) {
-  Box lb;
-  if (w.lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
-  else lb = child_box(T.groups, w.lg, w.lk);
-  w.hit = leaf_cast<C>(T.recs, w.llink, w.lcnt, lb, w.ray.o, w.ray.d, clip, w.ex, w.h, cc);
-  w.state = (w.hit || w.lg < 0) ? kWalkDone : kWalkTrace;
 }
 
 // castInLeaf for every pending lane of the wave at once, with all 64 lanes sharing
