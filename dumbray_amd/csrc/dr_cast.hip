@@ -14,7 +14,7 @@ constexpr int kCastBlock = 256;
 constexpr int kWavesPerBlock = kCastBlock / 64;
 constexpr int kWaveChunk = 128;       // rays a wave takes from the global counter at a time
 constexpr int kRefillMin = 16;        // idle lanes that trigger a refill
-constexpr int kLeafMin = 32;          // pending leaves that trigger the leaf tests
+constexpr int kLeafMin = 16;          // pending leaves that trigger the (wave-shared) leaf tests
 
 __device__ __forceinline__ void write_result(int r, bool hit, const Hit &h, int *__restrict__ out_id,
                                              float *__restrict__ out_t, float *__restrict__ out_p) {

@@ -22,7 +22,7 @@ namespace dr {
 constexpr int kRB = 256;               // lanes per workgroup
 constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per round
 constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
-constexpr int kLeafMin = 32;           // pending leaves that trigger the leaf tests
+constexpr int kLeafMin = 16;           // pending leaves that trigger the (wave-shared) leaf tests
 constexpr int kShadeMin = 32;          // finished casts that trigger the path updates
 // Waves per SIMD the render kernels are compiled for: the register cap this sets
 // (168 VGPRs) costs a few spills but beats 2 waves per SIMD clearly.
