@@ -9,6 +9,7 @@ from ._lib import DumbRayError, lib  # noqa: F401
 from .octree import OCTREE_MAX_DEPTH, OCTREE_POLYCOUNT_TO_SPLIT_NODE, DeviceScene, Octree  # noqa: F401
 from . import plugins  # noqa: F401
 from .render import BlockBasedFrameBuffer, DumbRenderer, RenderBatch  # noqa: F401
+from .scene_file import DumbRenderContext  # noqa: F401
 
 __all__ = ["DumbRayError", "lib", "Octree", "DeviceScene", "OCTREE_MAX_DEPTH", "OCTREE_POLYCOUNT_TO_SPLIT_NODE",
-           "BlockBasedFrameBuffer", "DumbRenderer", "RenderBatch", "plugins"]
+           "BlockBasedFrameBuffer", "DumbRenderer", "RenderBatch", "DumbRenderContext", "plugins"]

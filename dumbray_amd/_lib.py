@@ -43,6 +43,12 @@ def _declare(L):
         "dr_batch_run": (c_int, [vp, vp]),
         "dr_batch_get_tuning": (c_int, [vp, vp]),
         "dr_batch_destroy": (None, [vp]),
+        "dr_scene_file_load": (c_int, [c_char_p, ctypes.POINTER(vp)]),
+        "dr_scene_file_parse": (c_int, [c_char_p, c_char_p, ctypes.POINTER(vp)]),
+        "dr_scene_file_info": (c_int, [vp, vp, vp, vp, vp]),
+        "dr_scene_file_geometry": (c_int, [vp, vp, vp, vp]),
+        "dr_scene_file_tables": (c_int, [vp, vp, vp]),
+        "dr_scene_file_destroy": (None, [vp]),
     }
     for name, (res, args) in decl.items():
         f = getattr(L, name)

@@ -69,13 +69,21 @@ DR_HD V3 normalized(V3 a) {
   return a * il;
 }
 
-// Camera sample of subpixel (i, j) of pixel (px, py): DumbRenderer.cu:190-232 (HEIGHT
-// boxing), SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21 with
+// Pixel size of a boxing mode (DumbRenderer.cu:190-200).
+DR_HD float pixel_size(float W, float H, int boxing) {
+  if (boxing == DR_BOXING_WIDTH) return 1.0f / W;
+  if (boxing == DR_BOXING_MIN) return 1.0f / ((H <= W) ? H : W);
+  if (boxing == DR_BOXING_MAX) return 1.0f / ((H >= W) ? H : W);
+  return 1.0f / H;
+}
+
+// Camera sample of subpixel (i, j) of pixel (px, py): DumbRenderer.cu:190-232 (pixel
+// size by boxing mode), SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21 with
 // DumbRand::range = u*(max-min)+min) or DefaultPerspectiveLense, Camera transform
 // (Transform.impl.h:166-168).
 DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, float u0, float u1, V3 &o, V3 &d) {
   const float W = (float)P.width, H = (float)P.height;
-  const float ps = (1.0f / H);
+  const float ps = pixel_size(W, H, P.boxing);
   const float pdx = ps / ((float)P.fsaa_x), pdy = ps / ((float)P.fsaa_y);
   const float ppx = ((float)px - (W / 2.0f)) * ps;
   const float ppy = ((H / 2.0f) - (float)py) * ps;

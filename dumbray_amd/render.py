@@ -35,7 +35,10 @@ class RenderParams(ctypes.Structure):
                 ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
                 ("cam", ctypes.c_float * 13), ("materials", ctypes.c_void_p), ("lights", ctypes.c_void_p),
                 ("nmat", ctypes.c_int32), ("nlights", ctypes.c_int32), ("lens", ctypes.c_int32),
-                ("plugins", ctypes.c_int32)]
+                ("plugins", ctypes.c_int32), ("boxing", ctypes.c_int32)]
+
+
+BOXING = {"height": 0, "width": 1, "min": 2, "max": 3}   # DumbRenderer::BoxingMode (DumbRenderer.cuh:12-18)
 
 
 def _vp(x):
@@ -90,11 +93,38 @@ class BlockBasedFrameBuffer:
         return out
 
 
+def _tables(materials, lights):
+    """(material table (n,12), light table (n,16), nmat, nlights, plugin set) from lists of
+    plugin dicts or from tables already in dr_material / dr_light layout (scene files)."""
+    if isinstance(materials, np.ndarray):
+        mt = np.ascontiguousarray(materials, np.float32).reshape(-1, 12)
+        nmat = mt.shape[0]
+        mkinds = mt[:, 0].copy().view(np.int32)
+        mt = mt if nmat else np.zeros((1, 12), np.float32)
+    else:
+        mt, nmat = PL.material_table(materials), len(materials)
+        mkinds = np.array([PL.MATERIAL_KINDS[m["kind"]] for m in materials], np.int32)
+    if isinstance(lights, np.ndarray):
+        lt = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)
+        nlights = lt.shape[0]
+        lkinds = lt[:, 12].copy().view(np.int32)
+        lt = lt if nlights else np.zeros((1, 16), np.float32)
+    else:
+        lt, nlights = PL.light_table(lights), len(lights)
+        lkinds = np.array([PL.LIGHT_KINDS[l.get("kind", "quad")] for l in lights], np.int32)
+    reference = bool((mkinds >= PL.MATERIAL_KINDS["dumb_based"]).any() or (lkinds != 0).any())
+    return mt, lt, nmat, nlights, (PL.PLUGINS_REFERENCE if reference else PL.PLUGINS_BENCHMARK)
+
+
 class DumbRenderer:
+    """``materials`` / ``lights``: lists of plugin dicts (dumbray_amd.plugins) or tables in
+    dr_material (n,12) / dr_light (n,16) float32 layout; ``camera``: a Camera or 13 floats
+    (dr_render_params.cam); ``boxing``: "height", "width", "min" or "max"."""
+
     def __init__(self, scene: DeviceScene, mat_id, materials, lights, camera, width: int, height: int,
                  samples_per_pixel=(1, 1), max_bounces: int = 2, rr_start: int = 1 << 30, rr_survive: float = 1.0,
                  ignore_backfaces: bool = True, seed: int = 0x5EED, device="cuda", lens: str = "stochastic",
-                 normals=None):
+                 normals=None, boxing="height"):
         import torch
 
         self.scene = scene
@@ -110,11 +140,11 @@ class DumbRenderer:
               "dr_scene_set_materials")
         if normals is not None:
             scene.set_normals(normals)
-        self.lens = LENSES[lens]
-        self.plugins = PL.plugin_set(materials, lights)
-        self.material_table = torch.from_numpy(PL.material_table(materials)).to(device)
-        self.light_table = torch.from_numpy(PL.light_table(lights)).to(device)
-        self.nmat, self.nlights = len(materials), len(lights)
+        self.lens = LENSES[lens] if isinstance(lens, str) else int(lens)
+        self.boxing = BOXING[boxing] if isinstance(boxing, str) else int(boxing)
+        mt, lt, self.nmat, self.nlights, self.plugins = _tables(materials, lights)
+        self.material_table = torch.from_numpy(mt.copy()).to(device)
+        self.light_table = torch.from_numpy(lt.copy()).to(device)
 
     # parameters an optimiser may update in place
     @property
@@ -133,13 +163,13 @@ class DumbRenderer:
         p.cull_backfaces = int(self.ignore_backfaces)
         p.seed_lo, p.seed_hi = self.seed & 0xFFFFFFFF, (self.seed >> 32) & 0xFFFFFFFF
         p.request, p.iteration = int(request), int(iteration)
-        cam = self.camera.as_array()
+        cam = self.camera.as_array() if hasattr(self.camera, "as_array") else np.asarray(self.camera, np.float32)
         for k in range(13):
             p.cam[k] = float(cam[k])
         p.materials = self.material_table.data_ptr()
         p.lights = self.light_table.data_ptr()
         p.nmat, p.nlights = self.nmat, self.nlights
-        p.lens, p.plugins = self.lens, self.plugins
+        p.lens, p.plugins, p.boxing = self.lens, self.plugins, self.boxing
         return p
 
     def renderBlocksGPU(self, frame: BlockBasedFrameBuffer, start_block: int, end_block: int, iteration: int = 1,

@@ -140,8 +140,18 @@ enum {
                                 (DumbRenderer.cu:396-413); dr_render_blocks only */
 };
 
+/* Pixel boxing (DumbRenderer::BoxingMode, DumbRenderer.cuh:12-18): the pixel size is
+ * 1 / height, 1 / width, 1 / min(width, height) or 1 / max(width, height)
+ * (DumbRenderer.cu:190-200). */
+enum {
+  DR_BOXING_HEIGHT = 0,
+  DR_BOXING_WIDTH = 1,
+  DR_BOXING_MIN = 2,
+  DR_BOXING_MAX = 3
+};
+
 /* Per-call render parameters (DumbRenderer::PixelRenderProcess::SceneConfiguration,
- * DumbRenderer.cuh:80-101, plus the estimator's seeds).  Boxing is HEIGHT based. */
+ * DumbRenderer.cuh:80-101, plus the estimator's seeds). */
 typedef struct {
   int32_t width, height;          /* image size in pixels */
   int32_t fsaa_x, fsaa_y;         /* subpixel grid = samples per pixel per iteration */
@@ -158,6 +168,7 @@ typedef struct {
   int32_t nmat, nlights;          /* nlights <= 16 */
   int32_t lens;                   /* DR_STOCHASTIC_LENS or DR_PERSPECTIVE_LENS */
   int32_t plugins;                /* DR_PLUGINS_BENCHMARK or DR_PLUGINS_REFERENCE */
+  int32_t boxing;                 /* DR_BOXING_* (0 = HEIGHT, the reference's default) */
 } dr_render_params;
 
 /* Attach per-triangle material ids (host int32[ntris], ntris = the octree's triangle
@@ -220,6 +231,47 @@ int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tuning *tuning
 int dr_batch_run(dr_batch b, void *stream);
 int dr_batch_get_tuning(dr_batch b, dr_batch_tuning *out);
 void dr_batch_destroy(dr_batch b);
+
+/* ---------------------------------------------------------------- scene files */
+
+/* A scene loaded from the reference's scene description (.dumb, Dson text) and the
+ * Wavefront OBJ files it names: DumbRenderContext::fromFile / fromDson
+ * (DumbRenderContext.cu:310-371) with its include, materials, lights, groups, objects,
+ * camera and renderer sections (:641-787, :106-271, :476-640), MeshReader::readObj
+ * (MeshReader.impl.h) and PolyMesh::bake (PolyMesh.impl.h:140-152).  Host memory only.
+ * Supported plugin types: materials dumb_based, simple_stochastic, fresnel_color;
+ * lights simple_soft_directional, simple_spherical; lens simple_stochastic.  Textures,
+ * spotlights, the spherical_segment lens and primitive meshes are rejected with an
+ * error.  Thread, device and block settings are checked and ignored. */
+typedef struct dr_scene_file_s *dr_scene_file;
+
+/* Renderer settings and camera of a scene file (DumbRenderContextData::RendererSettings,
+ * DumbRenderContext.cu:43-50, defaults :282-289), in dr_render_params terms. */
+typedef struct {
+  int32_t boxing;            /* "pixel"/"boxing"; default DR_BOXING_HEIGHT */
+  int32_t max_bounces;       /* "pixel"/"bounces"; default 2 */
+  int32_t fsaa_x, fsaa_y;    /* "pixel"/"samples_per_pixel"; default 1 x 1 */
+  int32_t ignore_backfaces;  /* "pixel"/"ignore_backfaces"; default 1 */
+  int32_t lens;              /* DR_STOCHASTIC_LENS */
+  float cam[13];             /* dr_render_params.cam: camera transform and lens (default angle 64) */
+} dr_scene_file_settings;
+
+/* DumbRenderContext::fromFile (DumbRenderContext.cu:310-335): parse `path` and every
+ * file it includes or names; relative names resolve against the including file's
+ * directory first.  On error returns DR_ERR_ARG and dr_last_error() holds the message. */
+int dr_scene_file_load(const char *path, dr_scene_file *out);
+/* DumbRenderContext::fromDson on scene text; relative file names resolve against
+ * base_dir (may be NULL or "" for the working directory). */
+int dr_scene_file_parse(const char *text, const char *base_dir, dr_scene_file *out);
+/* Counts (triangles in push order, materials, lights) and settings; any pointer may be NULL. */
+int dr_scene_file_info(dr_scene_file f, dr_scene_file_settings *settings, int *ntris, int *nmat, int *nlights);
+/* Host arrays for dr_octree_build (tris[ntris*9]), dr_scene_set_normals
+ * (normals[ntris*9], BakedTriFace::norm) and dr_scene_set_materials (mat_id[ntris]);
+ * any pointer may be NULL. */
+int dr_scene_file_geometry(dr_scene_file f, float *tris, float *normals, int32_t *mat_id);
+/* Material and light tables (materials[nmat], lights[nlights]); either may be NULL. */
+int dr_scene_file_tables(dr_scene_file f, dr_material *materials, dr_light *lights);
+void dr_scene_file_destroy(dr_scene_file f);
 
 /* dr_cast plus per-ray work counters of the traversal that ran (measurement only;
  * no counterpart in the reference, which prints timings only): out_counts is a

@@ -282,7 +282,13 @@ struct Camera {
 
 class DumbRenderer {
  public:
-  enum BoxingMode { BOXING_MODE_HEIGHT_BASED = 0 };
+  // DumbRenderer::BoxingMode (DumbRenderer.cuh:12-18)
+  enum BoxingMode {
+    BOXING_MODE_HEIGHT_BASED = DR_BOXING_HEIGHT,
+    BOXING_MODE_WIDTH_BASED = DR_BOXING_WIDTH,
+    BOXING_MODE_MIN_BASED = DR_BOXING_MIN,
+    BOXING_MODE_MAX_BASED = DR_BOXING_MAX
+  };
   DumbRenderer(const Octree<> &geometry, const std::vector<dr_material> &materials,
                const std::vector<dr_light> &lights, const Camera &camera, int maxBounces = 2, int samplesPerPixelX = 1,
                int samplesPerPixelY = 1, bool ignoreBackfaces = true)
@@ -321,6 +327,7 @@ class DumbRenderer {
   }
   void setSeed(uint64_t seed) { seed_ = seed; }
   void setLens(int lens) { lens_ = lens; }   // DR_STOCHASTIC_LENS or DR_PERSPECTIVE_LENS
+  void setBoxingMode(BoxingMode mode) { boxing_ = mode; }
   dr_material *deviceMaterials() { return mats_; }
   dr_light *deviceLights() { return lights_; }
 
@@ -349,6 +356,7 @@ class DumbRenderer {
     p.nlights = nlights_;
     p.lens = lens_;
     p.plugins = plugins_;
+    p.boxing = boxing_;
     return p;
   }
   // DumbRenderer::renderBlocksGPU: blocks [startBlock, endBlock) of `iteration` into the device frame.
@@ -378,6 +386,72 @@ class DumbRenderer {
   bool ignoreBackfaces_ = true;
   uint64_t seed_ = 0x5EED;
   int lens_ = DR_STOCHASTIC_LENS, plugins_ = DR_PLUGINS_BENCHMARK;
+  int boxing_ = DR_BOXING_HEIGHT;
+};
+
+// DumbRenderContext (DumbRenderContext.cuh): a .dumb scene file loaded (dr_scene_file_*)
+// into an Octree of Renderable<BakedTriFace> with its materials, lights, camera and
+// renderer settings.  buildFromFile mirrors DumbRenderContext::buildFromFile
+// (DumbRenderContext.cu:295-301): on failure it returns false and writes the error to
+// errorStream when one is given.
+class DumbRenderContext {
+ public:
+  template <typename Stream>
+  bool buildFromFile(const std::string &filename, Stream *errorStream) {
+    dr_scene_file f = nullptr;
+    if (dr_scene_file_load(filename.c_str(), &f) != DR_OK) {
+      if (errorStream) (*errorStream) << dr_last_error() << "\n";
+      return false;
+    }
+    int ntris = 0, nmat = 0, nlights = 0;
+    dr_scene_file_info(f, &settings_, &ntris, &nmat, &nlights);
+    std::vector<float> tris(9 * (size_t)ntris), nrm(9 * (size_t)ntris);
+    std::vector<int32_t> mat(ntris);
+    materials_.assign(nmat, dr_material{});
+    lights_.assign(nlights, dr_light{});
+    dr_scene_file_geometry(f, tris.data(), nrm.data(), mat.data());
+    dr_scene_file_tables(f, materials_.data(), lights_.data());
+    dr_scene_file_destroy(f);
+    geometry_.reset();
+    for (int i = 0; i < ntris; i++) {
+      Renderable<BakedTriFace> r{};
+      const float *t = &tris[9 * (size_t)i], *n = &nrm[9 * (size_t)i];
+      r.object.vert = Triangle{{t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}};
+      r.object.norm = Triangle{{n[0], n[1], n[2]}, {n[3], n[4], n[5]}, {n[6], n[7], n[8]}};
+      r.materialId = mat[i];
+      geometry_.push(r);
+    }
+    geometry_.build();
+    normals_ = std::move(nrm);
+    if (ntris > 0) check(dr_scene_set_normals(geometry_.deviceScene(), normals_.data(), ntris), "dr_scene_set_normals");
+    return true;
+  }
+  bool buildFromFile(const std::string &filename) { return buildFromFile<std::ostream>(filename, nullptr); }
+
+  const Octree<> &geometry() const { return geometry_; }
+  const std::vector<dr_material> &materials() const { return materials_; }
+  const std::vector<dr_light> &lights() const { return lights_; }
+  const dr_scene_file_settings &settings() const { return settings_; }
+  Camera camera() const {
+    const float *c = settings_.cam;
+    return Camera{{c[0], c[1], c[2]}, {c[3], c[4], c[5]}, {c[6], c[7], c[8]}, {c[9], c[10], c[11]}, c[12]};
+  }
+  // A renderer with the file's settings (DumbRenderContext::runWindowRender,
+  // DumbRenderContext.cu:869-874).
+  DumbRenderer *newRenderer() const {
+    auto *r = new DumbRenderer(geometry_, materials_, lights_, camera(), settings_.max_bounces, settings_.fsaa_x,
+                               settings_.fsaa_y, settings_.ignore_backfaces != 0);
+    r->setLens(settings_.lens);
+    r->setBoxingMode((DumbRenderer::BoxingMode)settings_.boxing);
+    return r;
+  }
+
+ private:
+  Octree<> geometry_;
+  std::vector<dr_material> materials_;
+  std::vector<dr_light> lights_;
+  std::vector<float> normals_;
+  dr_scene_file_settings settings_{};
 };
 
 }  // namespace dumbray

@@ -84,7 +84,7 @@ class OraRenderCfg(ctypes.Structure):
                 ("fsaa_y", ctypes.c_int32), ("max_bounces", ctypes.c_int32), ("rr_start", ctypes.c_int32),
                 ("rr_survive", ctypes.c_float), ("cull_backfaces", ctypes.c_int32), ("seed_lo", ctypes.c_uint32),
                 ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
-                ("cam", ctypes.c_float * 13), ("lens", ctypes.c_int32)]
+                ("cam", ctypes.c_float * 13), ("lens", ctypes.c_int32), ("boxing", ctypes.c_int32)]
 
 
 KINDS = {"lambert": 0, "mirror": 1, "metal": 2, "dumb_based": 3, "simple_stochastic": 4, "fresnel": 5,
@@ -221,7 +221,7 @@ class Renderer:
 
     def __init__(self, tree, mat_id, materials, lights, cam_array, width, height, samples_per_pixel=(1, 1),
                  max_bounces=2, rr_start=1 << 30, rr_survive=1.0, ignore_backfaces=True, seed=0x5EED, lens=0,
-                 normals=None):
+                 normals=None, boxing=0):
         self.tree = tree
         self.mat_id = np.ascontiguousarray(mat_id, np.int32)
         self.mats = (OraMaterial * max(len(materials), 1))()
@@ -243,6 +243,7 @@ class Renderer:
         for k in range(13):
             cfg.cam[k] = float(cam_array[k])
         cfg.lens = int(lens)
+        cfg.boxing = int(boxing)
         self.cfg = cfg
 
     def set_albedo(self, i, rgb):

@@ -78,13 +78,21 @@ static void rng4(const rctx *C, uint32_t pix, uint32_t samp, uint32_t layer, uin
 	for (int k = 0; k < 4; k++) u[k] = (float)(o[k] >> 8) * (1.0f / 16777216.0f);
 }
 
+/* DumbRenderer::PixelRenderProcess::countPixelSize (DumbRenderer.cu:190-200) */
+static float pixel_size(float W, float H, int boxing) {
+	if (boxing == 1) return 1.0f / W;
+	if (boxing == 2) return 1.0f / ((H <= W) ? H : W);
+	if (boxing == 3) return 1.0f / ((H >= W) ? H : W);
+	return 1.0f / H;
+}
+
 /* Camera sample of subpixel (i, j) of pixel (px, py): DumbRenderer.cu:190-232 with
- * HEIGHT boxing, SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21,
+ * the boxing's pixel size, SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21,
  * DumbRand::range = u*(max-min)+min) or DefaultPerspectiveLense, Camera transform
  * (Camera.impl.cuh:7-10, Transform::direction Transform.impl.h:166-168). */
 static void camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, const float u[2], v3 *o, v3 *d) {
 	const float W = (float)c->width, H = (float)c->height;
-	const float ps = (1.0f / H);
+	const float ps = pixel_size(W, H, c->boxing);
 	const float pdx = ps / ((float)c->fsaa_x), pdy = ps / ((float)c->fsaa_y);
 	const float ppx = ((float)px - (W / 2.0f)) * ps;
 	const float ppy = ((H / 2.0f) - (float)py) * ps;

@@ -58,6 +58,7 @@ typedef struct {
 	int32_t iteration;              /* 1-based; samples of iteration k are numbered from (k-1)*fsaa_x*fsaa_y */
 	float cam[13];                  /* position, right, up, forward, focal (0.5 / tan(fov/2)) */
 	int32_t lens;                   /* 0: SimpleStochasticLense jitter, 1: DefaultPerspectiveLense */
+	int32_t boxing;                 /* pixel size 1/H, 1/W, 1/min(W,H), 1/max(W,H) for 0..3 */
 } ora_render_cfg;
 
 /* Pixel colours (row-major, rows [row0, row1) x W x 3) of one iteration, into out_rgb
