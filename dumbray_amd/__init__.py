@@ -10,6 +10,8 @@ from .octree import OCTREE_MAX_DEPTH, OCTREE_POLYCOUNT_TO_SPLIT_NODE, DeviceScen
 from . import plugins  # noqa: F401
 from .render import BlockBasedFrameBuffer, DumbRenderer, RenderBatch  # noqa: F401
 from .scene_file import DumbRenderContext  # noqa: F401
+from .render_process import BufferedRenderProcess  # noqa: F401
+from . import images  # noqa: F401
 
 __all__ = ["DumbRayError", "lib", "Octree", "DeviceScene", "OCTREE_MAX_DEPTH", "OCTREE_POLYCOUNT_TO_SPLIT_NODE",
-           "BlockBasedFrameBuffer", "DumbRenderer", "RenderBatch", "DumbRenderContext", "plugins"]
+           "BlockBasedFrameBuffer", "DumbRenderer", "RenderBatch", "DumbRenderContext", "BufferedRenderProcess", "images", "plugins"]

@@ -49,6 +49,8 @@ def _declare(L):
         "dr_scene_file_geometry": (c_int, [vp, vp, vp, vp]),
         "dr_scene_file_tables": (c_int, [vp, vp, vp]),
         "dr_scene_file_destroy": (None, [vp]),
+        "dr_frame_to_rgba8": (c_int, [vp, c_int, c_int, vp, vp]),
+        "dr_png_write": (c_int, [c_char_p, vp, c_int, c_int]),
     }
     for name, (res, args) in decl.items():
         f = getattr(L, name)

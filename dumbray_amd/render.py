@@ -63,12 +63,18 @@ class BlockBasedFrameBuffer:
     """Device BlockBasedFrameBuffer<8,16>: data is a float32 tensor (blocks*128, 4)."""
 
     def __init__(self, width: int, height: int, device="cuda"):
+        self.device = device
+        self.setResolution(width, height)
+
+    def setResolution(self, width: int, height: int) -> bool:
+        """FrameBuffer::setResolution: a new, zeroed frame of the given size."""
         import torch
 
         self.width, self.height = int(width), int(height)
         self.data = torch.zeros((block_count(self.width, self.height) * BLOCK_SIZE, 4), dtype=torch.float32,
-                                device=device)
+                                device=self.device)
         self._idx = torch.from_numpy(block_to_image_index(self.width, self.height)).to(self.data.device)
+        return True
 
     def getBlockSize(self) -> int:
         return BLOCK_SIZE
@@ -145,6 +151,39 @@ class DumbRenderer:
         mt, lt, self.nmat, self.nlights, self.plugins = _tables(materials, lights)
         self.material_table = torch.from_numpy(mt.copy()).to(device)
         self.light_table = torch.from_numpy(lt.copy()).to(device)
+
+    # ------------------------------------------------ BufferedRenderer / Renderer iteration interface
+    # (Renderer.cu:136-156, BufferedRenderer.cu:7): iterate() renders iteration k + 1 into the
+    # frame buffer, blending 1/(k + 1); the resolution follows the frame buffer.
+    def setFrameBuffer(self, frame):
+        if frame is not getattr(self, "_frame", None):
+            self._frame = frame
+            self.resetIterations()
+
+    def getFrameBuffer(self):
+        return getattr(self, "_frame", None)
+
+    def iteration(self) -> int:
+        return getattr(self, "_iteration", 0)
+
+    def resetIterations(self):
+        self._iteration = 0
+
+    def iterate(self) -> bool:
+        import torch
+
+        frame = self.getFrameBuffer()
+        if frame is None:
+            return False
+        self.width, self.height = frame.getSize()
+        k = self.iteration() + 1
+        try:
+            self.render(frame, k)
+            torch.cuda.current_stream(frame.data.device).synchronize()
+        except DumbRayError:
+            return False
+        self._iteration = k
+        return True
 
     # parameters an optimiser may update in place
     @property

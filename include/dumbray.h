@@ -273,6 +273,17 @@ int dr_scene_file_geometry(dr_scene_file f, float *tris, float *normals, int32_t
 int dr_scene_file_tables(dr_scene_file f, dr_material *materials, dr_light *lights);
 void dr_scene_file_destroy(dr_scene_file f);
 
+/* ---------------------------------------------------------------- image output */
+
+/* Images::saveBufferPNG's colour conversion (Images.cu:7-19, :28-32): frame (device
+ * BlockBasedFrameBuffer<8,16>, width x height) to out (device, row-major RGBA8,
+ * width*height*4 bytes), each channel 255*c clamped to [0, 255] and truncated (NaN -> 0).
+ * Asynchronous on `stream`. */
+int dr_frame_to_rgba8(const float *frame, int width, int height, uint8_t *out, void *stream);
+/* lodepng::encode as Images::saveBufferPNG calls it (Images.cu:33): an 8-bit RGBA PNG
+ * file of a host row-major RGBA8 image. */
+int dr_png_write(const char *path, const uint8_t *rgba, int width, int height);
+
 /* dr_cast plus per-ray work counters of the traversal that ran (measurement only;
  * no counterpart in the reference, which prints timings only): out_counts is a
  * device int32[n*4] receiving, per ray, the groups expanded (8 child box tests,

@@ -10,15 +10,22 @@
 //   DefaultPerspectiveLense / SimpleStochasticLense focal length and Camera transform
 //       (DefaultPerspectiveLense.impl.cuh:5-9, Camera.cuh:12-24)
 //   DumbRenderer::renderBlocksGPU (DumbRenderer.cuh:61-62) and Renderer::backward
+//   DumbRenderContext::buildFromFile (scene files), BufferedRenderProcess, Images::saveBufferPNG
 // Errors throw dumbray::Error carrying dr_last_error().
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
+#include <ctime>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dumbray.h"
@@ -359,6 +366,28 @@ class DumbRenderer {
     p.boxing = boxing_;
     return p;
   }
+  // BufferedRenderer / Renderer iteration interface (Renderer.cu:136-156, BufferedRenderer.cu:7):
+  // iterate() renders iteration k + 1 into the frame buffer (blending 1/(k + 1)) and waits for it;
+  // the resolution follows the frame buffer.
+  void setFrameBuffer(BlockBuffer *frame) {
+    if (frame != frame_) {
+      frame_ = frame;
+      resetIterations();
+    }
+  }
+  BlockBuffer *getFrameBuffer() const { return frame_; }
+  int iteration() const { return iteration_; }
+  void resetIterations() { iteration_ = 0; }
+  bool iterate() {
+    if (!frame_) return false;
+    int w, h;
+    frame_->getSize(&w, &h);
+    const dr_render_params p = params(w, h, iteration_ + 1);
+    if (dr_render_blocks(scene_, &p, frame_->deviceData(), 0, frame_->getBlockCount(), nullptr) != DR_OK) return false;
+    if (hipStreamSynchronize(nullptr) != hipSuccess) return false;
+    iteration_++;
+    return true;
+  }
   // DumbRenderer::renderBlocksGPU: blocks [startBlock, endBlock) of `iteration` into the device frame.
   bool renderBlocksGPU(BlockBuffer &frame, int startBlock, int endBlock, int iteration, hipStream_t stream = nullptr) {
     int w, h;
@@ -387,6 +416,169 @@ class DumbRenderer {
   uint64_t seed_ = 0x5EED;
   int lens_ = DR_STOCHASTIC_LENS, plugins_ = DR_PLUGINS_BENCHMARK;
   int boxing_ = DR_BOXING_HEIGHT;
+  BlockBuffer *frame_ = nullptr;
+  int iteration_ = 0;
+};
+
+// Images::saveBufferPNG (Images.cu:22-36): the frame as an 8-bit RGBA PNG, each channel
+// 255*c clamped and truncated; returns the reference's error codes.
+namespace Images {
+enum Error {
+  IMAGES_NO_ERROR = 0,
+  IMAGES_ERROR_FRAME_BUFFER_INVALID = 1,
+  IMAGES_ERROR_ALLOCATION_FAILED = 2,
+  IMAGES_ERROR_EXTERNAL_LIBRARY_FAILED = 3
+};
+inline Error saveBufferPNG(BlockBuffer &frame, const std::string &filename) {
+  int w, h;
+  frame.getSize(&w, &h);
+  if (w <= 0 || h <= 0) return IMAGES_ERROR_FRAME_BUFFER_INVALID;
+  uint8_t *dev = nullptr;
+  if (hipMalloc((void **)&dev, (size_t)w * h * 4) != hipSuccess) return IMAGES_ERROR_ALLOCATION_FAILED;
+  std::vector<uint8_t> host((size_t)w * h * 4);
+  const bool ok = dr_frame_to_rgba8(frame.deviceData(), w, h, dev, nullptr) == DR_OK &&
+                  hipMemcpy(host.data(), dev, host.size(), hipMemcpyDeviceToHost) == hipSuccess;
+  (void)hipFree(dev);
+  if (!ok) return IMAGES_ERROR_EXTERNAL_LIBRARY_FAILED;
+  return dr_png_write(filename.c_str(), host.data(), w, h) == DR_OK ? IMAGES_NO_ERROR
+                                                                    : IMAGES_ERROR_EXTERNAL_LIBRARY_FAILED;
+}
+}  // namespace Images
+
+// BufferedRenderProcess (BufferedRenderProcess.cu:16-229): a host thread that keeps
+// iterating a renderer into the back buffer until the target iteration count (-1 = no
+// limit), swapping front and back after each iteration when they differ, resizing the
+// back buffer to the target resolution (which resets the iterations), and calling the
+// iteration / render-completion / already-rendered / error callbacks.  No window: a
+// negative target resolution keeps the buffer's size.
+class BufferedRenderProcess {
+ public:
+  typedef void (*Callback)(void *);
+  BufferedRenderProcess() = default;
+  ~BufferedRenderProcess() { end(); }
+  BufferedRenderProcess(const BufferedRenderProcess &) = delete;
+  BufferedRenderProcess &operator=(const BufferedRenderProcess &) = delete;
+
+  void start() {
+    std::lock_guard<std::mutex> g(threadLock_);
+    if (started_) return;
+    {
+      std::lock_guard<std::mutex> s(settingsLock_);
+      kill_ = false;
+      started_ = true;
+    }
+    startClock_ = clock();
+    thread_ = std::thread(&BufferedRenderProcess::renderProcess, this);
+  }
+  void end() {
+    std::lock_guard<std::mutex> g(threadLock_);
+    if (!started_) return;
+    {
+      std::lock_guard<std::mutex> s(settingsLock_);
+      kill_ = true;
+    }
+    thread_.join();
+    renderClocks_ = renderTime();
+    std::lock_guard<std::mutex> s(settingsLock_);
+    started_ = false;
+  }
+  void lockSettings() { settingsLock_.lock(); }
+  void unlockSettings() { settingsLock_.unlock(); }
+  void setRenderer(DumbRenderer *r) { std::lock_guard<std::mutex> s(settingsLock_); renderer_ = r; }
+  void setBuffer(BlockBuffer *b) { std::lock_guard<std::mutex> s(settingsLock_); front_ = back_ = b; }
+  void setDoubleBuffers(BlockBuffer *a, BlockBuffer *b) { std::lock_guard<std::mutex> s(settingsLock_); front_ = a; back_ = b; }
+  void setTargetIterations(int n) { std::lock_guard<std::mutex> s(settingsLock_); iterations_ = n; }
+  void setInfinateTargetIterations() { setTargetIterations(-1); }
+  void setTargetResolution(int w, int h) { std::lock_guard<std::mutex> s(settingsLock_); tw_ = w; th_ = h; }
+  void setIterationCompletionCallback(Callback c, void *a) { std::lock_guard<std::mutex> s(settingsLock_); onIter_ = {c, a}; }
+  void setRenderCompletionCallback(Callback c, void *a) { std::lock_guard<std::mutex> s(settingsLock_); onDone_ = {c, a}; }
+  void setAlreadyRenderedCallback(Callback c, void *a) { std::lock_guard<std::mutex> s(settingsLock_); onAlready_ = {c, a}; }
+  void setErrorOnIteration(Callback c, void *a) { std::lock_guard<std::mutex> s(settingsLock_); onError_ = {c, a}; }
+  // Returns once the render thread has read the current settings.
+  void synchSettings() {
+    std::unique_lock<std::mutex> lock(settingsLock_);
+    if (!started_) return;
+    synchCond_.wait(lock);
+  }
+  long long renderTime() const { return renderClocks_ + (started_ ? (long long)(clock() - startClock_) : 0); }
+
+ private:
+  struct Cb {
+    Callback f = nullptr;
+    void *arg = nullptr;
+    void operator()() const { if (f) f(arg); }
+  };
+  void renderProcess() {
+    bool swap = false;
+    while (true) {
+      DumbRenderer *renderer;
+      BlockBuffer *front, *back;
+      int target, tw, th;
+      Cb onIter, onDone, onAlready, onError;
+      {
+        std::lock_guard<std::mutex> s(settingsLock_);
+        if (kill_) break;
+        renderer = renderer_;
+        front = swap ? back_ : front_;
+        back = swap ? front_ : back_;
+        target = iterations_;
+        tw = tw_;
+        th = th_;
+        onIter = onIter_;
+        onDone = onDone_;
+        onAlready = onAlready_;
+        onError = onError_;
+        synchCond_.notify_all();
+      }
+      if (back != nullptr && tw > 0 && th > 0) {
+        int w, h;
+        back->getSize(&w, &h);
+        if (w != tw || h != th) {
+          back->setResolution(tw, th);
+          if (renderer) {
+            renderer->resetIterations();
+            renderClocks_ = 0;
+            startClock_ = clock();
+          }
+        }
+      }
+      if (renderer == nullptr) {
+        std::this_thread::sleep_for(std::chrono::milliseconds(8));
+        continue;
+      }
+      renderer->setFrameBuffer(back);
+      if (renderer->iteration() != target) {
+        if (!renderer->iterate()) {
+          onError();
+          continue;
+        }
+        if (back != front) {
+          swap = !swap;
+          renderer->resetIterations();
+          renderClocks_ = 0;
+          startClock_ = clock();
+        }
+        onIter();
+        if (renderer->iteration() == target) onDone();
+      } else {
+        onAlready();
+        std::this_thread::sleep_for(std::chrono::milliseconds(1));
+      }
+    }
+    synchCond_.notify_all();
+  }
+
+  std::mutex threadLock_, settingsLock_;
+  std::condition_variable synchCond_;
+  std::thread thread_;
+  std::atomic<bool> started_{false};
+  bool kill_ = false;
+  DumbRenderer *renderer_ = nullptr;
+  BlockBuffer *front_ = nullptr, *back_ = nullptr;
+  int iterations_ = -1, tw_ = -1, th_ = -1;
+  Cb onIter_, onDone_, onAlready_, onError_;
+  std::atomic<long long> renderClocks_{0};
+  std::atomic<clock_t> startClock_{0};
 };
 
 // DumbRenderContext (DumbRenderContext.cuh): a .dumb scene file loaded (dr_scene_file_*)

@@ -409,3 +409,25 @@ def brute_cast(tris, rays, clip=False, exclude=None, nthreads=0):
     ex = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.int32)
     lib().ora_brute_cast(tris, int(tris.shape[0]), rays, n, int(clip), _ptr(ex), ids, t, _ptr(second), int(nthreads))
     return {"id": ids, "t": t, "second": second}
+
+
+def frame_bytes(img):
+    """Images::translateChannelFloatToByte (Images.cu:7-12) per channel of an (..., 4) float32
+    image: v = 255 * c in fp32; 255 when v > 255, 0 when v < 0 (or NaN), else (unsigned char)v."""
+    c = np.asarray(img, np.float32)
+    v = np.float32(255.0) * c
+    out = np.zeros(v.shape, np.uint8)
+    ok = (v >= 0) & (v <= 255)
+    out[ok] = v[ok].astype(np.uint8)          # truncation toward zero
+    out[v > 255] = 255
+    return out
+
+
+def blend(prev, img, k):
+    """Iteration k's frame: img * (1/k) + prev * (1 - 1/k) in fp32 (k = 1 sets the frame;
+    DumbRenderer.cu:64, BlockBasedFrameBuffer::blendBlockPixelColor)."""
+    img = np.asarray(img, np.float32)
+    if k == 1 or prev is None:
+        return img.copy()
+    a = np.float32(1.0) / np.float32(k)
+    return (img * a + np.asarray(prev, np.float32) * (np.float32(1.0) - a)).astype(np.float32)
