@@ -13,6 +13,7 @@ timed on this host as cpu_baseline.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -113,6 +114,60 @@ def bench_path_trace(D, S, stream, cpu_seconds, with_cpu):
                              "kind": "port", "sample": f"rows {r0}..{r0 + rows - 1} of the {W}x{H} frame at 16 spp "
                                                        f"through the oracle estimator, {spent:.1f} s"}
         log(f"path trace cpu oracle: {out['cpu_oracle']}")
+    return out
+
+
+def bench_scene_file(D, stream, with_cpu, cpu_seconds):
+    """The reference's example scene (tests/golden/example_scene_0: cargo.obj, 1670 tris, 9
+    reference-shader materials, 6 lights) loaded through the scene-file loader and rendered
+    at 1920x1080 with its own settings (1x1 spp, 2 bounces), then converted to RGBA8 bytes."""
+    import torch
+    path = os.path.join(ROOT, "tests", "golden", "example_scene_0", "0.dumb")
+    t0 = time.perf_counter()
+    ctx = D.DumbRenderContext.fromFile(path)
+    load_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    tree = ctx.octree()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    W, H = 1920, 1080
+    g = ctx.renderer(W, H, tree=tree)
+    frame = D.BlockBasedFrameBuffer(W, H)
+    ms = _event_ms(lambda: g.render(frame), 10, stream)
+    spp = ctx.samples_per_pixel[0] * ctx.samples_per_pixel[1]
+    out8 = torch.empty((H, W, 4), dtype=torch.uint8, device=frame.data.device)
+
+    def conv():
+        D.lib().dr_frame_to_rgba8(ctypes.c_void_p(frame.data.data_ptr()), W, H, ctypes.c_void_p(out8.data_ptr()),
+                                  ctypes.c_void_p(stream.cuda_stream))
+    conv_ms = _event_ms(conv, 50, stream)
+    conv_bytes = frame.data.numel() * 4 + out8.numel()
+    out = {"workload": f"example_scene_0 (cargo.obj, {ctx.triangle_count} tris, reference plugins) {W}x{H}, "
+                       f"{spp} spp, {ctx.max_bounces} bounces",
+           "load_ms": round(load_ms, 2), "octree_build_ms": round(build_ms, 2), "ms_per_frame": round(ms, 3),
+           "mpix_spp_s": round(W * H * spp / ms / 1e3, 2),
+           "frame_to_rgba8": {"ms": round(conv_ms, 4), "bytes": conv_bytes,
+                              "gb_s": round(conv_bytes / (conv_ms / 1e3) / 1e9, 1)}}
+    log(f"example scene 1080p: {ms:.3f} ms/frame, {out['mpix_spp_s']} Mpix*spp/s; rgba8 {conv_ms:.4f} ms")
+    if with_cpu:
+        import oracle
+        from oracle import scene_file as SF
+        sf = SF.load(path)
+        o = oracle.Renderer(oracle.Octree(sf.tris, 8, 24), sf.mat_id, sf.materials, sf.lights, sf.cam, W, H, sf.fsaa,
+                            sf.max_bounces, ignore_backfaces=sf.ignore_backfaces, normals=sf.normals,
+                            boxing=sf.boxing)
+        threads = min(os.cpu_count() or 1, 16)
+        rows = 16
+        while True:
+            r0 = H // 2 - rows // 2
+            t0 = time.perf_counter()
+            o.render_ref(nthreads=threads, rows=(r0, r0 + rows))
+            spent = time.perf_counter() - t0
+            if spent >= cpu_seconds / 8 or rows >= 1024:
+                break
+            rows *= 2
+        out["cpu_oracle"] = {"value": round(W * rows * spp / spent / 1e6, 4), "unit": "Mpix*spp/s", "cores": threads,
+                             "kind": "port", "sample": f"rows {r0}..{r0 + rows - 1} of the {W}x{H} frame through the "
+                                                       f"oracle's reference estimator, {spent:.2f} s"}
     return out
 
 
@@ -354,7 +409,8 @@ def main():
         render = {"path_trace_1080p": bench_path_trace(D, S, stream, args.cpu_seconds, not args.no_cpu),
                   "path_trace_256": bench_small(D, S, stream),
                   "adjoint_512": bench_adjoint(D, S, stream, args.adam_steps),
-                  "batch_64_requests": None if args.no_batch else bench_batch(D, S, stream)}
+                  "batch_64_requests": None if args.no_batch else bench_batch(D, S, stream),
+                  "scene_file_example_0": bench_scene_file(D, stream, not args.no_cpu, args.cpu_seconds)}
 
     line = {
         "metric": "closest-hit Mrays/s & Mpix·spp/s at 1080p; adjoint/forward time ratio",
