@@ -250,12 +250,62 @@ DR_HD bool sample_ref(const dr_render_params &P, const dr_material &M, const Ref
   return false;                                       // FresnelShader.impl.cuh:11-13: no samples
 }
 
-// getVertexPhotons of light L (index l) for p: photon ray (q, w), colour pc, and whether
-// w is unit length.  false: the light sends no photon.
-DR_HD bool photon_ref(const dr_render_params &P, const dr_light &L, int l, uint32_t pix, uint32_t samp, int layer,
-                      V3 p, V3 &q, V3 &w, float pc[3], bool &unit) {
+// Lights::Spotlight flags (Spotlight.cuh:28-32), kept in dr_light.reserved.
+constexpr int kSpotSampleMask = 31, kSpotCastShadows = 32, kSpotEmitFromSphere = 64;
+
+// Photons a light sends per vertex (Spotlight: its sample count, at most 16; others 1).
+DR_HD int light_samples(const dr_light &L) { return L.kind == DR_SPOT_LIGHT ? (L.reserved & kSpotSampleMask) : 1; }
+// Whether the light's photons are tested for shadows (Light::getVertexPhotons castShadows).
+DR_HD bool light_casts_shadows(const dr_light &L) { return L.kind != DR_SPOT_LIGHT || (L.reserved & kSpotCastShadows); }
+
+// getVertexPhotons of light L (index l), photon k of its samples, for p: photon ray
+// (q, w), colour pc, and whether w is unit length.  false: no photon.  Photon k of a
+// multi-sample light draws its uniforms with k in bits 16-19 of the Philox counter.
+DR_HD bool photon_ref(const dr_render_params &P, const dr_light &L, int l, int k, uint32_t pix, uint32_t samp,
+                      int layer, V3 p, V3 &q, V3 &w, float pc[3], bool &unit) {
   float u[4];
-  rng4(P, pix, samp, (uint32_t)layer, 16u + (uint32_t)l, u);
+  rng4(P, pix, samp, (uint32_t)layer | ((uint32_t)k << 8), 16u + (uint32_t)l, u);
+  if (L.kind == DR_SPOT_LIGHT) {                      // Lights::Spotlight, Spotlight.impl.cuh:41-90
+    const V3 dir = v3(L.edge_u[0], L.edge_u[1], L.edge_u[2]);
+    const V3 pos0 = v3(L.corner[0], L.corner[1], L.corner[2]);
+    const float size = L.radius, inner = L.edge_v[0], outer = L.edge_v[1], falloff = L.edge_v[2];
+    const bool sphere = (L.reserved & kSpotEmitFromSphere) != 0;
+    const V3 nondir = (dir.z > 0.9f) ? v3(1.0f, 0.0f, 0.0f) : v3(0.0f, 0.0f, 1.0f);
+    const V3 ver = normalized(cross(dir, nondir));
+    const V3 hor = cross(dir, ver);
+    V3 pos;
+    if (sphere) {
+      pos = point_on_sphere(u[0], u[1], size);
+      pos = pos * u[2];
+      pos = pos + pos0;
+    } else {
+      float sa, ca;
+      sincos_2pi(u[0], sa, ca);                       // angle = range(0, 2*PI)
+      pos = pos0 + (((ver * ca) + (hor * sa)) * size) * __builtin_sqrtf(u[1]);
+    }
+    const V3 delta = p - pos;
+    const float d2 = (delta.x * delta.x + delta.y * delta.y) + delta.z * delta.z;
+    if (d2 < kVecEps) return false;
+    const float dist = __builtin_sqrtf(d2);
+    const float base = size * size;
+    const float spread = __builtin_fmaxf(1.0f - outer, kVecEps);
+    const float surface = (sphere ? 0.0f : base) + (d2 * spread);
+    const float scale = base / (((float)(L.reserved & kSpotSampleMask)) * surface);
+    for (int c = 0; c < 3; c++) pc[c] = L.emission[c] * scale;
+    const V3 wd = delta * (1.0f / dist);
+    const float cosine = dot(wd, dir);
+    if (cosine < inner) {
+      if (cosine < outer) return false;
+      const float a = ((inner - cosine) / (inner - outer));
+      const float b = (1.0f - a);
+      const float f = pow_ref(((a * b) * b) + (b * (1.0f - (a * a))), falloff);
+      for (int c = 0; c < 3; c++) pc[c] *= f;
+    }
+    q = pos;
+    w = wd;
+    unit = true;
+    return true;
+  }
   if (L.kind == DR_DIRECTIONAL_LIGHT) {               // SimpleDirectionalLight.impl.cuh:13-16
     const V3 dir = v3(L.edge_u[0], L.edge_u[1], L.edge_u[2]);
     q = p - dir * L.dist;

@@ -56,7 +56,7 @@ __device__ __forceinline__ bool start_path(const dr_render_params &P, int block,
   const int i = s / P.fsaa_y, j = s - (s / P.fsaa_y) * P.fsaa_y;
   float u[4];
   rng4(P, st.pix, st.samp, 0, 0, u);
-  camera_ray(P, x, y, i, j, u[0], u[1], st.o, st.d);
+  camera_ray(P, x, y, i, j, u[0], u[1], u[2], u[3], st.o, st.d);
   st.layer = 0;
   st.phase = 0;
   st.restrict_tri = -1;
@@ -160,14 +160,14 @@ struct BenchPath {
 };
 
 // Path updates of the reference estimator (DR_PLUGINS_REFERENCE), DumbRenderer.cu:256-443:
-// every layer casts one photon per light; a photon that first reaches another surface
+// every layer casts each light's photons (one, or a spotlight's samples); a photon that first reaches another surface
 // continues through it with that surface's getReflectedColor (observer = the photon
 // direction) as its colour when that colour is not black (:396-413).  Colours are
 // carried forward: L += beta * (getReflectedColor of each direct photon), and the
 // child layer's throughput is beta * getReflectedColor(indirect photon) * sampleWeight.
 struct RefPath {
   uint32_t pix, samp;
-  int layer, phase, light, excl, trans;
+  int layer, phase, light, lk, excl, trans;   // lk: photons of the current light still to send
   V3 o, d;                          // the next ray to cast
   float beta[3], L[3];
   int h, mid, stype;                // current vertex
@@ -188,7 +188,7 @@ struct RefPath {
     const int i = s / P.fsaa_y, j = s - (s / P.fsaa_y) * P.fsaa_y;
     float u[4];
     rng4(P, pix, samp, 0, 0, u);
-    camera_ray(P, x, y, i, j, u[0], u[1], o, d);
+    camera_ray(P, x, y, i, j, u[0], u[1], u[2], u[3], o, d);
     layer = 0;
     phase = 0;
     excl = -1;
@@ -242,6 +242,7 @@ struct RefPath {
       for (int c = 0; c < 3; c++) bnext[c] = beta[c] * (F[c] * sw);
     }
     light = (M.kind == DR_MIRROR || M.kind == DR_METAL) ? P.nlights : -1;   // delta BSDFs take no direct light
+    lk = 0;
   }
   // Direct light of the current photon at the vertex.
   __device__ __forceinline__ void add_direct(const dr_render_params &P) {
@@ -286,13 +287,24 @@ struct RefPath {
         }
       }
     }
-    for (int l = light + 1; l < P.nlights; l++) {
+    // next photon: the current light's remaining samples (last first, DumbRenderer.cu:290-293),
+    // then the following lights; a light that casts no shadows illuminates at once (:306-315)
+    while (true) {
+      if (lk == 0) {
+        if (++light >= P.nlights) break;
+        lk = light_samples(P.lights[light]);
+        if (lk == 0) continue;
+      }
+      lk--;
       V3 q, wv;
       bool u1;
-      if (!photon_ref(P, P.lights[l], l, pix, samp, layer, H.p, q, wv, pc, u1)) continue;
-      light = l;
+      if (!photon_ref(P, P.lights[light], light, lk, pix, samp, layer, H.p, q, wv, pc, u1)) continue;
       pd = wv;
       unit = u1;
+      if (!light_casts_shadows(P.lights[light])) {
+        add_direct(P);
+        continue;
+      }
       o = q;
       d = wv;
       excl = -1;
@@ -348,7 +360,12 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
     w.state = kWalkDone;
     bool exhausted = false;
     auto finish = [&](const float *L) {
-      for (int c = 0; c < 3; c++) samp_col[3 * job + c] = L[c];
+      // getPixelColor: the layer colour (SimpleStochasticLense, DefaultPerspectiveLense) or the
+      // colour times the lens filter (SphericalSegmentLense.impl.cuh:31-34)
+      if (P.lens == DR_SPHERICAL_SEGMENT_LENS)
+        for (int c = 0; c < 3; c++) samp_col[3 * job + c] = L[c] * P.lens_p[4 + c];
+      else
+        for (int c = 0; c < 3; c++) samp_col[3 * job + c] = L[c];
       job = -1;
     };
     for (;;) {
@@ -635,7 +652,8 @@ static int check_params(dr_scene s, const dr_render_params *p) {
   if (!p->materials || p->nmat <= s->max_mat) return fail(DR_ERR_ARG, "render: material table smaller than the ids in use");
   if (!(p->rr_survive > 0.0f && p->rr_survive <= 1.0f)) return fail(DR_ERR_ARG, "render: rr_survive must be in (0, 1]");
   if (s->max_depth > kMaxTreeDepth) return fail(DR_ERR_ARG, "render: octree deeper than the traversal stack");
-  if (p->lens != DR_STOCHASTIC_LENS && p->lens != DR_PERSPECTIVE_LENS) return fail(DR_ERR_ARG, "render: unknown lens");
+  if (p->lens != DR_STOCHASTIC_LENS && p->lens != DR_PERSPECTIVE_LENS && p->lens != DR_SPHERICAL_SEGMENT_LENS)
+    return fail(DR_ERR_ARG, "render: unknown lens");
   if (p->plugins != DR_PLUGINS_BENCHMARK && p->plugins != DR_PLUGINS_REFERENCE)
     return fail(DR_ERR_ARG, "render: unknown plugin set");
   return DR_OK;
@@ -671,6 +689,7 @@ extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *d
   if (rc != DR_OK) return rc;
   if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return dr::fail(DR_ERR_ARG, "dr_backward: null buffers");
   if (p->plugins != DR_PLUGINS_BENCHMARK) return dr::fail(DR_ERR_ARG, "dr_backward: only the benchmark plugin set has an adjoint");
+  if (p->lens == DR_SPHERICAL_SEGMENT_LENS) return dr::fail(DR_ERR_ARG, "dr_backward: the spherical segment lens has no adjoint");
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const int maxv = p->max_bounces + 1;
   const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
@@ -822,6 +841,8 @@ extern "C" int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tun
     if (rc == DR_OK && !reqs[r].frame) rc = dr::fail(DR_ERR_ARG, "dr_batch_create: null frame");
     if (rc == DR_OK && reqs[r].params.plugins != DR_PLUGINS_BENCHMARK)
       rc = dr::fail(DR_ERR_ARG, "dr_batch_create: batches render the benchmark plugin set only");
+    if (rc == DR_OK && reqs[r].params.lens == DR_SPHERICAL_SEGMENT_LENS)
+      rc = dr::fail(DR_ERR_ARG, "dr_batch_create: batches do not take the spherical segment lens");
     if (rc != DR_OK) break;
     dr::DevRequest d;
     d.T = reqs[r].scene->tree;

@@ -783,11 +783,66 @@ struct dr_scene_file_s {
       l.corner[0] = pos.x; l.corner[1] = pos.y; l.corner[2] = pos.z;
       l.radius = radius;
     } else if (t.str == "spotlight") {
-      return error("Error: light type \"spotlight\" is not supported");
+      if (!spotlight(d, l)) return false;
     } else {
       return error("Error: Unknown light type: \"" + t.str + "\"");
     }
     lights.push_back(l);
+    return true;
+  }
+
+  // Lights::Spotlight::fromDson (Spotlight.impl.cuh:95-161) and its constructor (:7-38)
+  bool spotlight(const DValue &d, dr_light &l) {
+    F3 shade = f3(1.0f, 1.0f, 1.0f), pos = f3(0.0f, 0.0f, 0.0f), dir = f3(0.0f, 0.0f, 1.0f);
+    float lum = 1500.0f, inner = 16.0f, outer = 64.0f, falloff = 1.0f, size = 0.25f;
+    bool sphere = false, shadows = true;
+    int samples = 1;
+    if (d.has("color") && !vec_from(d.at("color"), shade, err)) return false;
+    if (!number_of(d, "luminosity", lum, "Error: Spotlight luminosity must be a number")) return false;
+    if (d.has("position") && !vec_from(d.at("position"), pos, err)) return false;
+    if (d.has("target")) {
+      F3 target{};
+      if (!vec_from(d.at("target"), target, err)) return false;
+      dir = sub(target, pos);
+    } else if (d.has("direction") && !vec_from(d.at("direction"), dir, err)) {
+      return false;
+    }
+    if (!number_of(d, "hotspot_angle", inner, "Error: Spotlight hotspot angle must be a number")) return false;
+    if (!number_of(d, "falloff_angle", outer, "Error: Spotlight falloff angle must be a number")) return false;
+    if (!number_of(d, "falloff_power", falloff, "Error: Spotlight falloff power must be a number")) return false;
+    if (!number_of(d, "emitter_size", size, "Error: Spotlight emitter size must be a number")) return false;
+    if (d.has("emitter_shape")) {
+      const DValue &e = d.at("emitter_shape");
+      if (e.kind != DValue::STRING) return error("Error: Spotlight emitter shape must be a string");
+      if (e.str == "disc") sphere = false;
+      else if (e.str == "sphere") sphere = true;     // any other name is reported and ignored
+    }
+    if (d.has("cast_shadows")) {
+      if (d.at("cast_shadows").kind != DValue::BOOL) return error("Error: Spotlight shadow cast flag must be a boolean");
+      shadows = d.at("cast_shadows").b;
+    }
+    if (d.has("sample_count")) {
+      if (d.at("sample_count").kind != DValue::NUMBER) return error("Error: Spotlight sample count must be a number");
+      samples = d.at("sample_count").i();
+    }
+    // constructor
+    const float kPi = 3.14159265359f, kEps = 0.000005f;
+    const F3 u = normalized(dir);
+    if (size < (32 * kEps)) size = (32 * kEps);
+    if (inner < 0.0f) inner = (-inner);
+    if (inner > 180.0f) inner = 180.0f;
+    if (outer < 0.0f) outer = (-inner);
+    if (outer > 180.0f) outer = 180.0f;
+    if (outer < inner) outer = inner;
+    l.kind = DR_SPOT_LIGHT;
+    l.emission[0] = shade.x * lum; l.emission[1] = shade.y * lum; l.emission[2] = shade.z * lum;
+    l.corner[0] = pos.x; l.corner[1] = pos.y; l.corner[2] = pos.z;
+    l.edge_u[0] = u.x; l.edge_u[1] = u.y; l.edge_u[2] = u.z;
+    l.edge_v[0] = std::cos(inner * kPi / 180.0f);
+    l.edge_v[1] = std::cos(outer * kPi / 180.0f);
+    l.edge_v[2] = falloff;
+    l.radius = size;
+    l.reserved = ((samples > 0) ? ((samples > 16) ? 16 : samples) : 0) | (shadows ? 32 : 0) | (sphere ? 64 : 0);
     return true;
   }
 
@@ -969,8 +1024,22 @@ struct dr_scene_file_s {
         lens_set = true;
         lens_x = x;
         settings.lens = DR_STOCHASTIC_LENS;
+        std::memset(settings.lens_p, 0, sizeof(settings.lens_p));
       } else if (t.str == "spherical_segment") {
-        return error("Error: lense type \"spherical_segment\" is not supported");
+        // Lenses::SphericalSegmentLense defaults (SphericalSegmentLense.cuh:9) and fromDson (.impl.cuh:38-70)
+        float ax = 60.0f, ay = 60.0f, dist = 128.0f, soft = 1.0f;
+        F3 filter = f3(1.0f, 1.0f, 1.0f);
+        if (l.has("angle")) {
+          if (l.at("angle").kind != DValue::NUMBER) return error("SimpleFocalPlaneLense angle must be a number");
+          ax = ay = l.at("angle").f();
+        }
+        if (!number_of(l, "distance", dist, "SimpleFocalPlaneLense distance must be a number")) return false;
+        if (!number_of(l, "sensor", soft, "SimpleFocalPlaneLense sensor size must be a number")) return false;
+        if (l.has("filter") && !vec_from(l.at("filter"), filter, err)) return false;
+        lens_set = true;
+        settings.lens = DR_SPHERICAL_SEGMENT_LENS;
+        const float lp[8] = {ax, ay, dist, soft, filter.x, filter.y, filter.z, 0.0f};
+        std::memcpy(settings.lens_p, lp, sizeof(lp));
       } else {
         return error("Error: Unknown lense type: \"" + t.str + "\"");
       }
@@ -1037,11 +1106,14 @@ struct dr_scene_file_s {
     return true;
   }
 
-  void fill_camera(float *cam) const {
+  void fill_camera(dr_scene_file_settings &st) const {
     const F3 o = camera.point(f3(0.0f, 0.0f, 0.0f));   // Camera::getPixelSamples: lens ray origin 0 moved by the transform
     const F3 r = camera.right(), u = camera.up(), f = camera.front();
     const float v[13] = {o.x, o.y, o.z, r.x, r.y, r.z, u.x, u.y, u.z, f.x, f.y, f.z, lens_x};
-    std::memcpy(cam, v, sizeof(v));
+    std::memcpy(st.cam, v, sizeof(v));
+    const float rows[9] = {camera.tx.x, camera.tx.y, camera.tx.z, camera.ty.x, camera.ty.y,
+                           camera.ty.z, camera.tz.x, camera.tz.y, camera.tz.z};
+    std::memcpy(st.cam_rows, rows, sizeof(rows));
   }
 };
 
@@ -1091,7 +1163,7 @@ int dr_scene_file_info(dr_scene_file f, dr_scene_file_settings *settings, int *n
   if (!f) return dr::fail(DR_ERR_ARG, "dr_scene_file_info: null scene file");
   if (settings) {
     *settings = f->settings;
-    f->fill_camera(settings->cam);
+    f->fill_camera(*settings);
   }
   if (ntris) *ntris = (int)f->geometry.size();
   if (nmat) *nmat = (int)f->materials.size();

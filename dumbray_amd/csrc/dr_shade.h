@@ -69,6 +69,35 @@ DR_HD V3 normalized(V3 a) {
   return a * il;
 }
 
+// sin and cos of an angle in radians: sincos_2pi of the angle in turns, reduced to [0, 1).
+DR_HD void sincos_rad(float a, float &s, float &c) {
+  float t = a * 0.159154943f;                 // 1 / (2 pi)
+  t = t - __builtin_floorf(t);
+  sincos_2pi(t, s, c);
+}
+
+// arcsin on [0, 1]: the Taylor series of arcsin (terms to x^21) for x <= 1/2, and
+// asin(x) = pi/2 - 2 asin(sqrt((1 - x) / 2)) above.
+DR_HD float asin_series(float x) {
+  const float z = x * x;
+  float p = 7.312525874e-03f;
+  p = p * z + 8.390335810e-03f;
+  p = p * z + 9.761609529e-03f;
+  p = p * z + 1.155180090e-02f;
+  p = p * z + 1.396484375e-02f;
+  p = p * z + 1.735276442e-02f;
+  p = p * z + 2.237215909e-02f;
+  p = p * z + 3.038194444e-02f;
+  p = p * z + 4.464285714e-02f;
+  p = p * z + 7.500000000e-02f;
+  p = p * z + 1.666666667e-01f;
+  return x + x * (z * p);
+}
+DR_HD float asin_unit(float x) {
+  if (x <= 0.5f) return asin_series(x);
+  return 1.57079637f - 2.0f * asin_series(__builtin_sqrtf((1.0f - x) * 0.5f));
+}
+
 // Pixel size of a boxing mode (DumbRenderer.cu:190-200).
 DR_HD float pixel_size(float W, float H, int boxing) {
   if (boxing == DR_BOXING_WIDTH) return 1.0f / W;
@@ -81,7 +110,8 @@ DR_HD float pixel_size(float W, float H, int boxing) {
 // size by boxing mode), SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21 with
 // DumbRand::range = u*(max-min)+min) or DefaultPerspectiveLense, Camera transform
 // (Transform.impl.h:166-168).
-DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, float u0, float u1, V3 &o, V3 &d) {
+DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, float u0, float u1, float u2, float u3,
+                      V3 &o, V3 &d) {
   const float W = (float)P.width, H = (float)P.height;
   const float ps = pixel_size(W, H, P.boxing);
   const float pdx = ps / ((float)P.fsaa_x), pdy = ps / ((float)P.fsaa_y);
@@ -90,6 +120,29 @@ DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, f
   const float sx = ppx + (((float)i) + 0.5f) * pdx;
   const float sy = ppy + (((float)j) + 0.5f) * pdy;
   float lx, ly, lz;
+  if (P.lens == DR_SPHERICAL_SEGMENT_LENS) {
+    // Lenses::SphericalSegmentLense (SphericalSegmentLense.impl.cuh:13-30): jittered angles
+    // alpha * position / RADIAN, target = direction * distance, origin on a disc of radius
+    // asin(u) * sensor; the origin goes through the camera's point map, the unit direction
+    // (target - origin) through Transform::direction.
+    const float offx = pdx * 0.5f, offy = pdy * 0.5f;
+    const float ha = ((sx + ((u0 * (offx - (-offx))) + (-offx))) * P.lens_p[0]) / 57.29577951308233f;
+    const float va = ((sy + ((u1 * (offy - (-offy))) + (-offy))) * P.lens_p[1]) / 57.29577951308233f;
+    float hs, hc, vs, vc;
+    sincos_rad(ha, hs, hc);
+    sincos_rad(va, vs, vc);
+    const V3 target = v3(hs * vc, vs, hc * vc) * P.lens_p[2];
+    float rs, rc;
+    sincos_2pi(u2, rs, rc);                     // range(0, 2*PI)
+    const V3 org = (v3(rc, rs, 0.0f) * asin_unit(u3)) * P.lens_p[3];
+    const V3 dl = normalized(target - org);
+    const float *R = P.cam_rows, *C = P.cam;
+    o = v3(((org.x * R[0] + org.y * R[1]) + org.z * R[2]) + C[0], ((org.x * R[3] + org.y * R[4]) + org.z * R[5]) + C[1],
+           ((org.x * R[6] + org.y * R[7]) + org.z * R[8]) + C[2]);
+    d = v3((dl.x * C[3] + dl.y * C[6]) + dl.z * C[9], (dl.x * C[4] + dl.y * C[7]) + dl.z * C[10],
+           (dl.x * C[5] + dl.y * C[8]) + dl.z * C[11]);
+    return;
+  }
   if (P.lens == DR_PERSPECTIVE_LENS) {
     // DefaultPerspectiveLense (DefaultPerspectiveLense.impl.cuh:12-15): subpixel centre,
     // direction normalized in lens space before the camera transform

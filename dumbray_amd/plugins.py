@@ -12,7 +12,9 @@ does, in fp32, so the table holds the same values the reference object would:
               directional_light                 SimpleDirectionalLight.impl.cuh:7-11
               soft_directional_light            SimpleSoftDirectionalLight.impl.cuh:5-10
               spherical_light                   SphericalLightEmitter.impl.cuh:5-9
-  lenses      STOCHASTIC_LENS (SimpleStochasticLense), PERSPECTIVE_LENS (DefaultPerspectiveLense)
+              spot_light                        Spotlight.impl.cuh:7-38
+  lenses      STOCHASTIC_LENS (SimpleStochasticLense), PERSPECTIVE_LENS (DefaultPerspectiveLense),
+              SPHERICAL_SEGMENT_LENS (spherical_segment_lens: SphericalSegmentLense.impl.cuh:6-34)
 
 Textures are not supported: every ColoredTexture is its colour.
 """
@@ -22,8 +24,9 @@ import numpy as np
 
 MATERIAL_KINDS = {"lambert": 0, "mirror": 1, "metal": 2, "dumb_based": 3, "simple_stochastic": 4, "fresnel": 5,
                   "default": 6}
-LIGHT_KINDS = {"quad": 0, "area": 0, "directional": 1, "soft_directional": 2, "spherical": 3}
-STOCHASTIC_LENS, PERSPECTIVE_LENS = 0, 1
+LIGHT_KINDS = {"quad": 0, "area": 0, "directional": 1, "soft_directional": 2, "spherical": 3, "spot": 4}
+SPOT_CAST_SHADOWS, SPOT_EMIT_FROM_SPHERE = 32, 64
+STOCHASTIC_LENS, PERSPECTIVE_LENS, SPHERICAL_SEGMENT_LENS = 0, 1, 2
 PLUGINS_BENCHMARK, PLUGINS_REFERENCE = 0, 1
 
 f32 = np.float32
@@ -124,6 +127,62 @@ def spherical_light(color=(1.0, 1.0, 1.0), position=(0.0, 0.0, 0.0), radius=1.0)
     return {"kind": "spherical", "emission": _rgb(color), "corner": tuple(position), "radius": float(f32(radius))}
 
 
+def spot_light(color=(1.0, 1.0, 1.0), luminosity=1500.0, position=(0.0, 0.0, 0.0), direction=(0.0, 0.0, 1.0),
+               inner_angle=16.0, outer_angle=64.0, falloff_power=1.0, emitter_size=0.25, emit_from_sphere=False,
+               cast_shadows=True, samples=1):
+    """Lights::Spotlight(shade, lum, pos, dir, innerAngle, outerAngle, falloffPower, emitSize,
+    emitFromSphere, castShadows, samples) as its constructor stores it (Spotlight.impl.cuh:7-38):
+    colour * luminosity, unit direction, emitter size at least 32*VECTOR_EPSILON, angles folded
+    into [0, 180] with outer >= inner, their cosines (C library cosf), samples clamped to [0, 16]."""
+    emission = tuple(float(f32(c) * f32(luminosity)) for c in color)
+    size = f32(emitter_size)
+    if size < f32(32) * f32(0.000005):
+        size = f32(32) * f32(0.000005)
+    inner, outer = f32(inner_angle), f32(outer_angle)
+    if inner < 0:
+        inner = -inner
+    if inner > 180:
+        inner = f32(180)
+    if outer < 0:
+        outer = -inner
+    if outer > 180:
+        outer = f32(180)
+    if outer < inner:
+        outer = inner
+    pi = f32(3.14159265359)
+    ci = _cosf((inner * pi) / f32(180))
+    co = _cosf((outer * pi) / f32(180))
+    n = int(samples)
+    flags = (16 if n > 16 else n) if n > 0 else 0
+    flags |= SPOT_CAST_SHADOWS if cast_shadows else 0
+    flags |= SPOT_EMIT_FROM_SPHERE if emit_from_sphere else 0
+    return {"kind": "spot", "emission": emission, "corner": tuple(float(f32(c)) for c in position),
+            "edge_u": _normalized(direction), "edge_v": (float(ci), float(co), float(f32(falloff_power))),
+            "radius": float(size), "flags": flags}
+
+
+def _cosf(x):
+    import ctypes
+    import ctypes.util
+
+    global _LIBM
+    if _LIBM is None:
+        _LIBM = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        _LIBM.cosf.restype = ctypes.c_float
+        _LIBM.cosf.argtypes = [ctypes.c_float]
+    return f32(_LIBM.cosf(float(x)))
+
+
+_LIBM = None
+
+
+def spherical_segment_lens(angle=(60.0, 60.0), distance=128.0, sensor=1.0, filter=(1.0, 1.0, 1.0)):
+    """Lenses::SphericalSegmentLense(angle, focalDistance, softness, sensitivity)
+    (SphericalSegmentLense.impl.cuh:6-11) as dr_render_params.lens_p."""
+    ax, ay = (angle, angle) if np.isscalar(angle) else angle
+    return np.array([ax, ay, distance, sensor, filter[0], filter[1], filter[2], 0.0], np.float32)
+
+
 def is_reference_light(l) -> bool:
     return LIGHT_KINDS[l.get("kind", "quad")] != 0
 
@@ -151,6 +210,7 @@ def light_table(lights) -> np.ndarray:
         t[i, 12] = np.array([LIGHT_KINDS[l.get("kind", "quad")]], np.int32).view(np.float32)[0]
         t[i, 13] = l.get("dist", 0.0)
         t[i, 14] = l.get("radius", 0.0)
+        t[i, 15] = np.array([int(l.get("flags", 0))], np.int32).view(np.float32)[0]
     return t
 
 

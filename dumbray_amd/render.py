@@ -22,7 +22,8 @@ from .octree import DeviceScene
 
 DR_LAMBERT, DR_MIRROR, DR_METAL = 0, 1, 2
 KINDS = PL.MATERIAL_KINDS
-LENSES = {"stochastic": PL.STOCHASTIC_LENS, "perspective": PL.PERSPECTIVE_LENS}
+LENSES = {"stochastic": PL.STOCHASTIC_LENS, "perspective": PL.PERSPECTIVE_LENS,
+          "spherical_segment": PL.SPHERICAL_SEGMENT_LENS}
 BLOCK_W, BLOCK_H = 8, 16
 BLOCK_SIZE = BLOCK_W * BLOCK_H
 DUMB_RENDERER_BOUNCE_LIMIT = 64      # DumbRenderer.cuh:7
@@ -35,7 +36,8 @@ class RenderParams(ctypes.Structure):
                 ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
                 ("cam", ctypes.c_float * 13), ("materials", ctypes.c_void_p), ("lights", ctypes.c_void_p),
                 ("nmat", ctypes.c_int32), ("nlights", ctypes.c_int32), ("lens", ctypes.c_int32),
-                ("plugins", ctypes.c_int32), ("boxing", ctypes.c_int32)]
+                ("plugins", ctypes.c_int32), ("boxing", ctypes.c_int32), ("lens_p", ctypes.c_float * 8),
+                ("cam_rows", ctypes.c_float * 9)]
 
 
 BOXING = {"height": 0, "width": 1, "min": 2, "max": 3}   # DumbRenderer::BoxingMode (DumbRenderer.cuh:12-18)
@@ -130,7 +132,7 @@ class DumbRenderer:
     def __init__(self, scene: DeviceScene, mat_id, materials, lights, camera, width: int, height: int,
                  samples_per_pixel=(1, 1), max_bounces: int = 2, rr_start: int = 1 << 30, rr_survive: float = 1.0,
                  ignore_backfaces: bool = True, seed: int = 0x5EED, device="cuda", lens: str = "stochastic",
-                 normals=None, boxing="height"):
+                 normals=None, boxing="height", lens_params=None, cam_rows=None):
         import torch
 
         self.scene = scene
@@ -148,6 +150,10 @@ class DumbRenderer:
             scene.set_normals(normals)
         self.lens = LENSES[lens] if isinstance(lens, str) else int(lens)
         self.boxing = BOXING[boxing] if isinstance(boxing, str) else int(boxing)
+        # SphericalSegmentLense parameters (plugins.spherical_segment_lens) and the camera's point-map
+        # rows (default: right, up, forward of an unscaled camera)
+        self.lens_params = np.zeros(8, np.float32) if lens_params is None else np.asarray(lens_params, np.float32)
+        self.cam_rows = None if cam_rows is None else np.asarray(cam_rows, np.float32)
         mt, lt, self.nmat, self.nlights, self.plugins = _tables(materials, lights)
         self.material_table = torch.from_numpy(mt.copy()).to(device)
         self.light_table = torch.from_numpy(lt.copy()).to(device)
@@ -205,6 +211,11 @@ class DumbRenderer:
         cam = self.camera.as_array() if hasattr(self.camera, "as_array") else np.asarray(self.camera, np.float32)
         for k in range(13):
             p.cam[k] = float(cam[k])
+        rows = self.cam_rows if self.cam_rows is not None else cam[3:12]
+        for k in range(9):
+            p.cam_rows[k] = float(rows[k])
+        for k in range(8):
+            p.lens_p[k] = float(self.lens_params[k])
         p.materials = self.material_table.data_ptr()
         p.lights = self.light_table.data_ptr()
         p.nmat, p.nlights = self.nmat, self.nlights

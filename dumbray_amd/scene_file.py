@@ -25,7 +25,7 @@ BOXING_NAMES = {0: "height", 1: "width", 2: "min", 3: "max"}
 class SceneFileSettings(ctypes.Structure):
     _fields_ = [("boxing", ctypes.c_int32), ("max_bounces", ctypes.c_int32), ("fsaa_x", ctypes.c_int32),
                 ("fsaa_y", ctypes.c_int32), ("ignore_backfaces", ctypes.c_int32), ("lens", ctypes.c_int32),
-                ("cam", ctypes.c_float * 13)]
+                ("cam", ctypes.c_float * 13), ("lens_p", ctypes.c_float * 8), ("cam_rows", ctypes.c_float * 9)]
 
 
 class DumbRenderContext:
@@ -38,6 +38,8 @@ class DumbRenderContext:
         self.camera = np.zeros(13, np.float32)
         self.max_bounces, self.samples_per_pixel, self.boxing = 2, (1, 1), "height"
         self.ignore_backfaces, self.lens = True, 0
+        self.lens_params = np.zeros(8, np.float32)
+        self.cam_rows = np.zeros(9, np.float32)
 
     # ------------------------------------------------------------ loading
     @classmethod
@@ -78,6 +80,8 @@ class DumbRenderContext:
             c.boxing = BOXING_NAMES.get(int(st.boxing), "height")
             c.ignore_backfaces = bool(st.ignore_backfaces)
             c.lens = int(st.lens)
+            c.lens_params = np.array(st.lens_p[:], np.float32)
+            c.cam_rows = np.array(st.cam_rows[:], np.float32)
             return c
         finally:
             L.dr_scene_file_destroy(h)
@@ -103,7 +107,8 @@ class DumbRenderContext:
 
         tree = tree if tree is not None else self.octree()
         kw = dict(samples_per_pixel=self.samples_per_pixel, max_bounces=self.max_bounces,
-                  ignore_backfaces=self.ignore_backfaces, lens=self.lens, boxing=self.boxing)
+                  ignore_backfaces=self.ignore_backfaces, lens=self.lens, boxing=self.boxing,
+                  lens_params=self.lens_params, cam_rows=self.cam_rows)
         kw.update(overrides)
         return DumbRenderer(tree.upload(), self.mat_id, self.material_table, self.light_table, self.camera, width,
                             height, seed=seed, device=device, normals=self.normals, **kw)

@@ -112,14 +112,19 @@ enum {
   DR_AREA_LIGHT = 0,              /* one-sided quad emitting on the (edge_u x edge_v) side */
   DR_DIRECTIONAL_LIGHT = 1,       /* SimpleDirectionalLight, SimpleDirectionalLight.impl.cuh:7-16 */
   DR_SOFT_DIRECTIONAL_LIGHT = 2,  /* SimpleSoftDirectionalLight, SimpleSoftDirectionalLight.impl.cuh:5-19 */
-  DR_SPHERICAL_LIGHT = 3          /* SphericalLightEmitter, SphericalLightEmitter.impl.cuh:5-18 */
+  DR_SPHERICAL_LIGHT = 3,         /* SphericalLightEmitter, SphericalLightEmitter.impl.cuh:5-18 */
+  DR_SPOT_LIGHT = 4               /* Lights::Spotlight, Spotlight.impl.cuh:7-90 */
 };
 
 /* One light (64 B).  Fields by kind:
  *   AREA: corner, edge_u, edge_v, emission (radiance scale).
  *   DIRECTIONAL: emission = colour, edge_u = unit direction, dist = distance of the photon origin.
  *   SOFT_DIRECTIONAL: as DIRECTIONAL, plus radius = distance * softness.
- *   SPHERICAL: emission = colour, corner = centre, radius. */
+ *   SPHERICAL: emission = colour, corner = centre, radius.
+ *   SPOT: emission = colour * luminosity, corner = position, edge_u = unit direction,
+ *     edge_v = (cos inner angle, cos outer angle, falloff power), radius = emitter size,
+ *     reserved = sample count (0-16) | 32 (casts shadows) | 64 (emits from a sphere, else a disc);
+ *     the values the Spotlight constructor stores (Spotlight.impl.cuh:7-38). */
 typedef struct {
   float corner[3], edge_u[3], edge_v[3], emission[3];
   int32_t kind;
@@ -130,7 +135,9 @@ typedef struct {
 /* Lenses (Lense plugins, Lense.cuh:46-82). */
 enum {
   DR_STOCHASTIC_LENS = 0,  /* SimpleStochasticLense: jitter inside the subpixel, SimpleStochasticLense.impl.cuh:11-25 */
-  DR_PERSPECTIVE_LENS = 1  /* DefaultPerspectiveLense: subpixel centre, unit direction, DefaultPerspectiveLense.impl.cuh:12-19 */
+  DR_PERSPECTIVE_LENS = 1, /* DefaultPerspectiveLense: subpixel centre, unit direction, DefaultPerspectiveLense.impl.cuh:12-19 */
+  DR_SPHERICAL_SEGMENT_LENS = 2  /* Lenses::SphericalSegmentLense: angular pixel grid, disc aperture and a colour filter,
+                                    SphericalSegmentLense.impl.cuh:6-34; needs lens_p and cam_rows; dr_render_blocks only */
 };
 
 /* Plugin sets. */
@@ -169,6 +176,10 @@ typedef struct {
   int32_t lens;                   /* DR_STOCHASTIC_LENS or DR_PERSPECTIVE_LENS */
   int32_t plugins;                /* DR_PLUGINS_BENCHMARK or DR_PLUGINS_REFERENCE */
   int32_t boxing;                 /* DR_BOXING_* (0 = HEIGHT, the reference's default) */
+  float lens_p[8];                /* DR_SPHERICAL_SEGMENT_LENS: angle x, angle y (degrees), focal distance,
+                                     sensor size, filter r, g, b, unused */
+  float cam_rows[9];              /* the camera Transform's rows trans.x, trans.y, trans.z (scale applied): the
+                                     point map of lens origins (Transform.impl.h:85-87); used by that lens */
 } dr_render_params;
 
 /* Attach per-triangle material ids (host int32[ntris], ntris = the octree's triangle
@@ -240,9 +251,8 @@ void dr_batch_destroy(dr_batch b);
  * camera and renderer sections (:641-787, :106-271, :476-640), MeshReader::readObj
  * (MeshReader.impl.h) and PolyMesh::bake (PolyMesh.impl.h:140-152).  Host memory only.
  * Supported plugin types: materials dumb_based, simple_stochastic, fresnel_color;
- * lights simple_soft_directional, simple_spherical; lens simple_stochastic.  Textures,
- * spotlights, the spherical_segment lens and primitive meshes are rejected with an
- * error.  Thread, device and block settings are checked and ignored. */
+ * lights simple_soft_directional, simple_spherical, spotlight; lenses simple_stochastic,
+ * spherical_segment.  Textures and primitive meshes are rejected with an error.  Thread, device and block settings are checked and ignored. */
 typedef struct dr_scene_file_s *dr_scene_file;
 
 /* Renderer settings and camera of a scene file (DumbRenderContextData::RendererSettings,
@@ -252,8 +262,10 @@ typedef struct {
   int32_t max_bounces;       /* "pixel"/"bounces"; default 2 */
   int32_t fsaa_x, fsaa_y;    /* "pixel"/"samples_per_pixel"; default 1 x 1 */
   int32_t ignore_backfaces;  /* "pixel"/"ignore_backfaces"; default 1 */
-  int32_t lens;              /* DR_STOCHASTIC_LENS */
+  int32_t lens;              /* DR_STOCHASTIC_LENS or DR_SPHERICAL_SEGMENT_LENS */
   float cam[13];             /* dr_render_params.cam: camera transform and lens (default angle 64) */
+  float lens_p[8];           /* dr_render_params.lens_p */
+  float cam_rows[9];         /* dr_render_params.cam_rows */
 } dr_scene_file_settings;
 
 /* DumbRenderContext::fromFile (DumbRenderContext.cu:310-335): parse `path` and every

@@ -275,6 +275,31 @@ inline dr_light sphericalLight(Vector3 color = {1.0f, 1.0f, 1.0f}, Vector3 posit
   l.radius = radius;
   return l;
 }
+// Lights::Spotlight(shade, lum, pos, dir, innerAngle, outerAngle, falloffPower, emitSize,
+// emitFromSphere, castShadows, samples) as its constructor stores it (Spotlight.impl.cuh:7-38).
+inline dr_light spotlight(Vector3 color = {1.0f, 1.0f, 1.0f}, float luminosity = 1500.0f,
+                          Vector3 position = {0.0f, 0.0f, 0.0f}, Vector3 direction = {0.0f, 0.0f, 1.0f},
+                          float innerAngle = 16.0f, float outerAngle = 64.0f, float falloffPower = 1.0f,
+                          float emitSize = 0.25f, bool emitFromSphere = false, bool castShadows = true, int samples = 1) {
+  dr_light l{};
+  l.kind = DR_SPOT_LIGHT;
+  l.emission[0] = color.x * luminosity; l.emission[1] = color.y * luminosity; l.emission[2] = color.z * luminosity;
+  l.corner[0] = position.x; l.corner[1] = position.y; l.corner[2] = position.z;
+  const float inv = 1.0f / std::sqrt(direction.x * direction.x + direction.y * direction.y + direction.z * direction.z);
+  l.edge_u[0] = direction.x * inv; l.edge_u[1] = direction.y * inv; l.edge_u[2] = direction.z * inv;
+  if (emitSize < (32 * 0.000005f)) emitSize = (32 * 0.000005f);
+  if (innerAngle < 0.0f) innerAngle = (-innerAngle);
+  if (innerAngle > 180.0f) innerAngle = 180.0f;
+  if (outerAngle < 0.0f) outerAngle = (-innerAngle);
+  if (outerAngle > 180.0f) outerAngle = 180.0f;
+  if (outerAngle < innerAngle) outerAngle = innerAngle;
+  l.edge_v[0] = std::cos(innerAngle * 3.14159265359f / 180.0f);
+  l.edge_v[1] = std::cos(outerAngle * 3.14159265359f / 180.0f);
+  l.edge_v[2] = falloffPower;
+  l.radius = emitSize;
+  l.reserved = ((samples > 0) ? ((samples > 16) ? 16 : samples) : 0) | (castShadows ? 32 : 0) | (emitFromSphere ? 64 : 0);
+  return l;
+}
 }  // namespace Plugins
 
 // Camera = lens focal length + Transform (position and world axes).

@@ -84,12 +84,13 @@ class OraRenderCfg(ctypes.Structure):
                 ("fsaa_y", ctypes.c_int32), ("max_bounces", ctypes.c_int32), ("rr_start", ctypes.c_int32),
                 ("rr_survive", ctypes.c_float), ("cull_backfaces", ctypes.c_int32), ("seed_lo", ctypes.c_uint32),
                 ("seed_hi", ctypes.c_uint32), ("request", ctypes.c_uint32), ("iteration", ctypes.c_int32),
-                ("cam", ctypes.c_float * 13), ("lens", ctypes.c_int32), ("boxing", ctypes.c_int32)]
+                ("cam", ctypes.c_float * 13), ("lens", ctypes.c_int32), ("boxing", ctypes.c_int32),
+                ("lens_p", ctypes.c_float * 8), ("cam_rows", ctypes.c_float * 9)]
 
 
 KINDS = {"lambert": 0, "mirror": 1, "metal": 2, "dumb_based": 3, "simple_stochastic": 4, "fresnel": 5,
          "default": 6}
-LIGHT_KINDS = {"quad": 0, "area": 0, "directional": 1, "soft_directional": 2, "spherical": 3}
+LIGHT_KINDS = {"quad": 0, "area": 0, "directional": 1, "soft_directional": 2, "spherical": 3, "spot": 4}
 
 
 def pack_material(m, out=None):
@@ -114,6 +115,7 @@ def pack_light(l, out=None):
         r.emission[c] = l.get("emission", (0, 0, 0))[c]
     r.dist = float(l.get("dist", 0.0))
     r.radius = float(l.get("radius", 0.0))
+    r.reserved = int(l.get("flags", 0))
     return r
 
 
@@ -131,13 +133,16 @@ def _bind_render(L):
     L.ora_sincos_2pi.argtypes = [ctypes.c_float, vp, vp]
     L.ora_pow.restype = ctypes.c_float
     L.ora_pow.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.ora_asin.restype = ctypes.c_float
+    L.ora_asin.argtypes = [ctypes.c_float]
+    L.ora_camera_ray.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp]
     L.ora_point_on_sphere.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, vp]
     L.ora_reflected_color.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     L.ora_indirect_sample.restype = ctypes.c_int
     L.ora_indirect_sample.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp, vp]
     L.ora_light_photon.restype = ctypes.c_int
-    L.ora_light_photon.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, vp, vp,
-                                   vp]
+    L.ora_light_photon.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_int, vp, vp, vp]
 
 
 def _flat32(a, n=None):
@@ -149,6 +154,20 @@ def _flat32(a, n=None):
 def pow_f(x, y):
     """The oracle's pow (the reference shaders' pow calls)."""
     return lib().ora_pow(ctypes.c_float(x), ctypes.c_float(y))
+
+
+def camera_ray(renderer, px, py, i, j, u):
+    """The oracle's camera ray (origin, direction) of a Renderer's settings for lens uniforms u[4]."""
+    uu = _flat32(u, 4)
+    o, d = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().ora_camera_ray(ctypes.addressof(renderer.cfg), int(px), int(py), int(i), int(j), uu.ctypes.data,
+                         o.ctypes.data, d.ctypes.data)
+    return o, d
+
+
+def asin_f(x):
+    """The oracle's arcsin on [0, 1] (SphericalSegmentLense aperture radius)."""
+    return lib().ora_asin(ctypes.c_float(x))
 
 
 def point_on_sphere(ua, ub, radius=1.0):
@@ -190,14 +209,15 @@ def indirect_sample(material, tri, normals, p, d, pix=0, samp=0, layer=0, seed=0
     return (out, ty.value) if ok else None
 
 
-def light_photon(light, p, index=0, pix=0, samp=0, layer=0, seed=0x5EED):
-    """getVertexPhotons of a light toward p: (origin, direction, colour) or None."""
+def light_photon(light, p, index=0, pix=0, samp=0, layer=0, seed=0x5EED, sample=0):
+    """getVertexPhotons of a light toward p (photon `sample` of a multi-sample light):
+    (origin, direction, colour) or None."""
     L = pack_light(light)
     pp = _flat32(p, 3)
     cfg = _cfg(seed)
     o, d, c = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
-    ok = lib().ora_light_photon(ctypes.addressof(L), int(index), pp.ctypes.data, ctypes.addressof(cfg), pix, samp,
-                                layer, o.ctypes.data, d.ctypes.data, c.ctypes.data)
+    ok = lib().ora_light_photon(ctypes.addressof(L), int(index), int(sample), pp.ctypes.data, ctypes.addressof(cfg),
+                                pix, samp, layer, o.ctypes.data, d.ctypes.data, c.ctypes.data)
     return (o, d, c) if ok else None
 
 
@@ -221,7 +241,7 @@ class Renderer:
 
     def __init__(self, tree, mat_id, materials, lights, cam_array, width, height, samples_per_pixel=(1, 1),
                  max_bounces=2, rr_start=1 << 30, rr_survive=1.0, ignore_backfaces=True, seed=0x5EED, lens=0,
-                 normals=None, boxing=0):
+                 normals=None, boxing=0, lens_params=None, cam_rows=None):
         self.tree = tree
         self.mat_id = np.ascontiguousarray(mat_id, np.int32)
         self.mats = (OraMaterial * max(len(materials), 1))()
@@ -244,6 +264,12 @@ class Renderer:
             cfg.cam[k] = float(cam_array[k])
         cfg.lens = int(lens)
         cfg.boxing = int(boxing)
+        rows = cam_array[3:12] if cam_rows is None else cam_rows
+        for k in range(9):
+            cfg.cam_rows[k] = float(rows[k])
+        if lens_params is not None:
+            for k in range(8):
+                cfg.lens_p[k] = float(lens_params[k])
         self.cfg = cfg
 
     def set_albedo(self, i, rgb):

@@ -90,7 +90,35 @@ static float pixel_size(float W, float H, int boxing) {
  * the boxing's pixel size, SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21,
  * DumbRand::range = u*(max-min)+min) or DefaultPerspectiveLense, Camera transform
  * (Camera.impl.cuh:7-10, Transform::direction Transform.impl.h:166-168). */
-static void camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, const float u[2], v3 *o, v3 *d) {
+/* sin/cos of an angle in radians through ora_sincos_2pi of the angle in turns reduced to [0, 1) */
+static void sincos_rad(float a, float *s, float *c) {
+	float t = a * 0.159154943f;
+	t = t - floorf(t);
+	ora_sincos_2pi(t, s, c);
+}
+
+/* arcsin on [0, 1]: Taylor series of arcsin to x^21 for x <= 1/2, asin(x) = pi/2 - 2 asin(sqrt((1-x)/2)) above */
+static float asin_series(float x) {
+	const float z = x * x;
+	float p = 7.312525874e-03f;
+	p = p * z + 8.390335810e-03f;
+	p = p * z + 9.761609529e-03f;
+	p = p * z + 1.155180090e-02f;
+	p = p * z + 1.396484375e-02f;
+	p = p * z + 1.735276442e-02f;
+	p = p * z + 2.237215909e-02f;
+	p = p * z + 3.038194444e-02f;
+	p = p * z + 4.464285714e-02f;
+	p = p * z + 7.500000000e-02f;
+	p = p * z + 1.666666667e-01f;
+	return x + x * (z * p);
+}
+float ora_asin(float x) {
+	if (x <= 0.5f) return asin_series(x);
+	return 1.57079637f - 2.0f * asin_series(sqrtf((1.0f - x) * 0.5f));
+}
+
+static void camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, const float u[4], v3 *o, v3 *d) {
 	const float W = (float)c->width, H = (float)c->height;
 	const float ps = pixel_size(W, H, c->boxing);
 	const float pdx = ps / ((float)c->fsaa_x), pdy = ps / ((float)c->fsaa_y);
@@ -99,6 +127,26 @@ static void camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, co
 	const float sx = ppx + (((float)i) + 0.5f) * pdx;
 	const float sy = ppy + (((float)j) + 0.5f) * pdy;
 	float lx, ly, lz;
+	if (c->lens == 2) {
+		/* Lenses::SphericalSegmentLense::getPixelSamples (SphericalSegmentLense.impl.cuh:13-30),
+		 * then Ray >>= Transform (Transform.impl.h:85-87, 166-168) */
+		const float offx = pdx * 0.5f, offy = pdy * 0.5f;
+		const float ha = ((sx + ((u[0] * (offx - (-offx))) + (-offx))) * c->lens_p[0]) / 57.29577951308233f;
+		const float va = ((sy + ((u[1] * (offy - (-offy))) + (-offy))) * c->lens_p[1]) / 57.29577951308233f;
+		float hs, hc, vs, vc, rs, rc;
+		sincos_rad(ha, &hs, &hc);
+		sincos_rad(va, &vs, &vc);
+		const v3 target = vmul(V(hs * vc, vs, hc * vc), c->lens_p[2]);
+		ora_sincos_2pi(u[2], &rs, &rc);
+		const v3 org = vmul(vmul(V(rc, rs, 0.0f), ora_asin(u[3])), c->lens_p[3]);
+		const v3 dl = vnormalized(vsub(target, org));
+		const float *R = c->cam_rows, *C = c->cam;
+		*o = V(((org.x * R[0] + org.y * R[1]) + org.z * R[2]) + C[0], ((org.x * R[3] + org.y * R[4]) + org.z * R[5]) + C[1],
+		       ((org.x * R[6] + org.y * R[7]) + org.z * R[8]) + C[2]);
+		*d = V((dl.x * C[3] + dl.y * C[6]) + dl.z * C[9], (dl.x * C[4] + dl.y * C[7]) + dl.z * C[10],
+		       (dl.x * C[5] + dl.y * C[8]) + dl.z * C[11]);
+		return;
+	}
 	if (c->lens == 1) {
 		/* DefaultPerspectiveLense (DefaultPerspectiveLense.impl.cuh:12-15): the subpixel centre,
 		 * direction normalized in lens space before the camera transform */
@@ -113,6 +161,13 @@ static void camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, co
 	const float *P = c->cam, *R = c->cam + 3, *U = c->cam + 6, *F = c->cam + 9;
 	*o = V(P[0], P[1], P[2]);
 	*d = V((lx * R[0] + ly * U[0]) + lz * F[0], (lx * R[1] + ly * U[1]) + lz * F[1], (lx * R[2] + ly * U[2]) + lz * F[2]);
+}
+
+void ora_camera_ray(const ora_render_cfg *c, int px, int py, int i, int j, const float u[4], float o[3], float d[3]) {
+	v3 ov, dv;
+	camera_ray(c, px, py, i, j, u, &ov, &dv);
+	o[0] = ov.x; o[1] = ov.y; o[2] = ov.z;
+	d[0] = dv.x; d[1] = dv.y; d[2] = dv.z;
 }
 
 /* geometric normal of triangle h, unit, facing the incoming ray; also the unit ray direction */

@@ -39,15 +39,17 @@ typedef struct {
 	int32_t reserved[2];
 } ora_material;
 
-enum { ORA_AREA_LIGHT = 0, ORA_DIRECTIONAL_LIGHT = 1, ORA_SOFT_DIRECTIONAL_LIGHT = 2, ORA_SPHERICAL_LIGHT = 3 };
+enum { ORA_AREA_LIGHT = 0, ORA_DIRECTIONAL_LIGHT = 1, ORA_SOFT_DIRECTIONAL_LIGHT = 2, ORA_SPHERICAL_LIGHT = 3,
+       ORA_SPOT_LIGHT = 4 };
 
 /* Same layout as the library's dr_light (64 B). */
 typedef struct {
 	float corner[3], edge_u[3], edge_v[3], emission[3];   /* area: emits on the (edge_u x edge_v) side */
 	int32_t kind;
 	float dist, radius;     /* directional: edge_u = unit direction, dist; soft: radius = dist * softness;
-	                           spherical: corner = centre, radius */
-	int32_t reserved;
+	                           spherical: corner = centre, radius; spot: corner = position, edge_u = unit
+	                           direction, edge_v = (cos inner, cos outer, falloff), radius = emitter size */
+	int32_t reserved;       /* spot: sample count | 32 casts shadows | 64 emits from a sphere */
 } ora_light;
 
 typedef struct {
@@ -59,6 +61,8 @@ typedef struct {
 	float cam[13];                  /* position, right, up, forward, focal (0.5 / tan(fov/2)) */
 	int32_t lens;                   /* 0: SimpleStochasticLense jitter, 1: DefaultPerspectiveLense */
 	int32_t boxing;                 /* pixel size 1/H, 1/W, 1/min(W,H), 1/max(W,H) for 0..3 */
+	float lens_p[8];                /* lens 2 (SphericalSegmentLense): angle x, y, distance, sensor, filter rgb */
+	float cam_rows[9];              /* camera Transform rows trans.x/y/z: point map of lens-2 origins */
 } ora_render_cfg;
 
 /* Pixel colours (row-major, rows [row0, row1) x W x 3) of one iteration, into out_rgb
@@ -88,6 +92,9 @@ int ora_render_ref(const ora_octree *t, const int32_t *mat_id, const ora_materia
 
 /* Probes of the reference plugins for tests.  photon_type: 0 direct, 1 indirect. */
 float ora_pow(float x, float y);
+float ora_asin(float x);          /* arcsin on [0, 1] as the SphericalSegmentLense aperture uses it */
+/* camera ray of subpixel (i, j) of pixel (px, py) for the lens uniforms u (probe for tests) */
+void ora_camera_ray(const ora_render_cfg *cfg, int px, int py, int i, int j, const float u[4], float o[3], float d[3]);
 void ora_point_on_sphere(float ua, float ub, float radius, float out[3]);
 void ora_reflected_color(const ora_material *m, const float tri[9], const float nrm[9], const float p[3],
                          const float photon_dir[3], const float photon_color[3], const float observer[3],
@@ -98,7 +105,7 @@ int ora_indirect_sample(const ora_material *m, const float tri[9], const float n
                         const float d[3], const ora_render_cfg *cfg, uint32_t pix, uint32_t samp, int layer,
                         float dir[3], int *type);
 /* getVertexPhotons of light l for the point p; returns 1 with the photon ray and colour, or 0. */
-int ora_light_photon(const ora_light *L, int l, const float p[3], const ora_render_cfg *cfg, uint32_t pix,
+int ora_light_photon(const ora_light *L, int l, int k, const float p[3], const ora_render_cfg *cfg, uint32_t pix,
                      uint32_t samp, int layer, float origin[3], float dir[3], float color[3]);
 
 /* Philox4x32-10 block and the sin/cos of 2*pi*u used by the samplers (probes for tests). */

@@ -600,6 +600,56 @@ def light_from(d):
         radius = _number(d, "radius", radius)
         return {"kind": "spherical", "emission": tuple(float(c) for c in shade),
                 "corner": tuple(float(c) for c in pos), "radius": float(radius)}
+    if t == "spotlight":                  # Spotlight.impl.cuh:95-161, constructor :7-38
+        shade, pos, direction = V(1, 1, 1), V(0, 0, 0), V(0, 0, 1)
+        if "color" in d:
+            shade = vector_from(d["color"])
+        lum = _number(d, "luminosity", f32(1500))
+        if "position" in d:
+            pos = vector_from(d["position"])
+        if "target" in d:
+            direction = vsub(vector_from(d["target"]), pos)
+        elif "direction" in d:
+            direction = vector_from(d["direction"])
+        inner = _number(d, "hotspot_angle", f32(16))
+        outer = _number(d, "falloff_angle", f32(64))
+        power = _number(d, "falloff_power", f32(1))
+        size = _number(d, "emitter_size", f32(0.25))
+        sphere = False
+        if "emitter_shape" in d:
+            if not isinstance(d["emitter_shape"], Str):
+                raise SceneError("Spotlight emitter shape must be a string")
+            if d["emitter_shape"] in ("disc", "sphere"):
+                sphere = d["emitter_shape"] == "sphere"
+        shadows = True
+        if "cast_shadows" in d:
+            if not isinstance(d["cast_shadows"], bool):
+                raise SceneError("Spotlight shadow cast flag must be a boolean")
+            shadows = d["cast_shadows"]
+        samples = 1
+        if "sample_count" in d:
+            if not _is_num(d["sample_count"]):
+                raise SceneError("Spotlight sample count must be a number")
+            samples = _iv(d["sample_count"])
+        eps32 = f32(32) * f32(0.000005)
+        if size < eps32:
+            size = eps32
+        if inner < 0:
+            inner = -inner
+        if inner > 180:
+            inner = f32(180)
+        if outer < 0:
+            outer = -inner
+        if outer > 180:
+            outer = f32(180)
+        if outer < inner:
+            outer = inner
+        pi = f32(3.14159265359)
+        flags = ((16 if samples > 16 else samples) if samples > 0 else 0) | (32 if shadows else 0) | (64 if sphere else 0)
+        return {"kind": "spot", "emission": tuple(float(c * lum) for c in shade), "corner": tuple(float(c) for c in pos),
+                "edge_u": tuple(float(c) for c in vnormalized(direction)),
+                "edge_v": (float(cosf((inner * pi) / f32(180))), float(cosf((outer * pi) / f32(180))), float(power)),
+                "radius": float(size), "flags": flags}
     raise SceneError('unsupported or unknown light type: "' + t + '"')
 
 
@@ -625,6 +675,7 @@ class SceneFile:
         self.camera = Transform()
         self.lens_x = None
         self.max_bounces, self.fsaa, self.boxing, self.ignore_backfaces = 2, (1, 1), 0, True
+        self.lens, self.lens_params = 0, [f32(0)] * 8
 
     # -------------------------------------------------------------- files
     def load(self, path, depth=0):
@@ -850,14 +901,28 @@ class SceneFile:
                 raise SceneError("Lense has to have a type")
             if not isinstance(lz["type"], Str):
                 raise SceneError("Lense type has to be a string")
-            if lz["type"] != "simple_stochastic":
-                raise SceneError('unsupported or unknown lense type: "' + lz["type"] + '"')
-            x = lens_focal(60.0)
-            if "angle" in lz:
-                if not _is_num(lz["angle"]):
-                    raise SceneError("SimpleStochasticLense angle can only be a number")
-                x = lens_focal(_fv(lz["angle"]))
-            self.lens_x = x
+            if lz["type"] == "simple_stochastic":
+                x = lens_focal(60.0)
+                if "angle" in lz:
+                    if not _is_num(lz["angle"]):
+                        raise SceneError("SimpleStochasticLense angle can only be a number")
+                    x = lens_focal(_fv(lz["angle"]))
+                self.lens_x = x
+                self.lens, self.lens_params = 0, [f32(0)] * 8
+            elif lz["type"] == "spherical_segment":    # SphericalSegmentLense.cuh:9, .impl.cuh:38-70
+                ax = ay = f32(60)
+                if "angle" in lz:
+                    if not _is_num(lz["angle"]):
+                        raise SceneError("SimpleFocalPlaneLense angle must be a number")
+                    ax = ay = _fv(lz["angle"])
+                dist = _number(lz, "distance", f32(128))
+                soft = _number(lz, "sensor", f32(1))
+                filt = vector_from(lz["filter"]) if "filter" in lz else V(1, 1, 1)
+                if self.lens_x is None:
+                    self.lens_x = f32(0)
+                self.lens, self.lens_params = 2, [ax, ay, dist, soft, filt[0], filt[1], filt[2], f32(0)]
+            else:
+                raise SceneError('unknown lense type: "' + lz["type"] + '"')
         if "transform" in d:
             self.camera = transform_from(d["transform"])
 
@@ -921,6 +986,11 @@ class SceneFile:
     @property
     def mat_id(self):
         return np.array([u[2] for u in self.units], np.int32)
+
+    @property
+    def cam_rows(self):
+        t = self.camera
+        return np.array(list(t.tx) + list(t.ty) + list(t.tz), np.float32)
 
     @property
     def cam(self):

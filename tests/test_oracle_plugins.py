@@ -309,3 +309,135 @@ def test_reference_estimator_equals_benchmark_estimator_on_benchmark_plugins(ora
     b = R.render_ref(nthreads=4)
     assert a.max() > 0
     assert np.allclose(a, b, rtol=2e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------- Lights::Spotlight
+def test_spot_constructor_follows_the_reference():
+    s = PL.spot_light((1, 0.5, 0.25), 100.0, (1, 2, 3), (0, -2, 0), inner_angle=-30, outer_angle=10, emitter_size=0,
+                      samples=40, cast_shadows=False, emit_from_sphere=True)
+    assert s["emission"] == (100.0, 50.0, 25.0) and s["edge_u"] == (0.0, -1.0, 0.0)
+    ci, co, _ = s["edge_v"]
+    # inner folded to 30, outer < inner -> inner
+    assert abs(ci - math.cos(math.radians(30))) < 1e-6 and co == ci
+    assert s["radius"] == float(np.float32(32) * np.float32(0.000005))
+    assert s["flags"] == 16 | PL.SPOT_EMIT_FROM_SPHERE
+    s = PL.spot_light(outer_angle=-5.0, inner_angle=20.0)          # negative outer -> -inner -> inner
+    assert s["edge_v"][1] == s["edge_v"][0]
+    s = PL.spot_light(inner_angle=200.0, outer_angle=300.0, samples=0)
+    assert s["edge_v"][0] == s["edge_v"][1] and abs(s["edge_v"][0] + 1.0) < 1e-6 and s["flags"] == PL.SPOT_CAST_SHADOWS
+
+
+def test_spot_photon_on_axis_closed_form(oracle):
+    """A point on the axis at distance D of a disc spot: colour E * s^2 / (n * (s^2 + D^2 (1 - cos outer))),
+    direction the axis, no falloff inside the hotspot."""
+    D_, E, size, n = 10.0, (100.0, 200.0, 300.0), 0.01, 4
+    s = PL.spot_light(E, 1.0, (0, 5, 0), (0, -1, 0), 20.0, 40.0, 1.0, size, samples=n)
+    base = np.float64(np.float32(size)) ** 2
+    want = np.array(E) * base / (n * (base + D_ * D_ * (1 - math.cos(math.radians(40)))))
+    for k in range(n):
+        o, d, c = oracle.light_photon(s, (0, 5 - D_, 0), pix=3, sample=k)
+        assert np.linalg.norm(o - np.array([0, 5, 0])) <= size * (1 + 1e-6)
+        assert np.allclose(d, (0, -1, 0), atol=2e-3)
+        assert np.allclose(c, want, rtol=1e-3)
+    # distinct samples draw distinct emitter points
+    pts = {tuple(oracle.light_photon(s, (0, -5, 0), pix=3, sample=k)[0]) for k in range(n)}
+    assert len(pts) == n
+
+
+def test_spot_photon_falloff_and_cone(oracle):
+    inner, outer, power = 20.0, 40.0, 2.5
+    s = PL.spot_light((1, 1, 1), 1.0, (0, 0, 0), (0, 0, 1), inner, outer, power, 1e-6)
+    for ang in (10.0, 25.0, 30.0, 39.0, 45.0, 90.0, 170.0):
+        p = (math.sin(math.radians(ang)) * 10, 0.0, math.cos(math.radians(ang)) * 10)
+        r = oracle.light_photon(s, p)
+        if ang > outer:
+            assert r is None
+            continue
+        _, d, c = r
+        cosine = math.cos(math.radians(ang))
+        base = np.float64(np.float32(32) * np.float32(0.000005)) ** 2
+        val = base / (base + 100.0 * (1 - math.cos(math.radians(outer))))
+        if ang > inner:
+            ci, co = math.cos(math.radians(inner)), math.cos(math.radians(outer))
+            a = (ci - cosine) / (ci - co)
+            b = 1 - a
+            val *= (a * b * b + b * (1 - a * a)) ** power
+        assert np.allclose(c, val, rtol=2e-4), (ang, c, val)
+
+
+def test_spot_sphere_emitter(oracle):
+    s = PL.spot_light((2, 2, 2), 1.0, (1, 1, 1), (1, 0, 0), 30.0, 60.0, 1.0, 0.5, emit_from_sphere=True, samples=2)
+    for pix in range(40):
+        o, d, c = oracle.light_photon(s, (11, 1, 1), pix=pix, sample=pix % 2)
+        assert np.linalg.norm(o - 1.0) <= 0.5 * (1 + 1e-6)
+        delta = np.array([11.0, 1, 1]) - o
+        # sphere emitters have no disc term: E * s^2 / (n * d^2 * (1 - cos outer))
+        want = 2.0 * 0.25 / (2 * delta.dot(delta) * (1 - math.cos(math.radians(60))))
+        assert np.allclose(c, want, rtol=1e-4)
+
+
+def test_spot_without_shadows_ignores_blockers(oracle):
+    """A spot that casts no shadows lights the floor through an opaque sheet; with shadows
+    the sheet blocks it; without the sheet both agree."""
+    W, H = 16, 12
+    floor = PL.dumb_based_shader((0.0, 0.0, 0.0), 8.0, (0.8, 0.6, 0.4), 0.0)
+    tris_open = _floor(0.0)
+    tris = np.concatenate([_floor(0.0), _floor(2.0, half=0.5)])
+    kw = dict(max_bounces=0, ignore_backfaces=False)
+    # light off to the side, so the sheet's shadow falls on floor the camera sees
+    lit = lambda shadows: PL.spot_light((1, 1, 1), 50.0, (1.0, 4, 0), (0, -1, 0), 70.0, 85.0, 1.0, 0.05,
+                                        cast_shadows=shadows, samples=3)
+    a = _render_ref(oracle, tris_open, [0, 0], [floor], [lit(True)], _camera_down(3.0), W, H, **kw)
+    b = _render_ref(oracle, tris_open, [0, 0], [floor], [lit(False)], _camera_down(3.0), W, H, **kw)
+    assert a.max() > 0 and np.array_equal(a, b)
+    blocker = PL.lambert((0.5, 0.5, 0.5))
+    c = _render_ref(oracle, tris, [0, 0, 1, 1], [floor, blocker], [lit(True)], _camera_down(3.0), W, H, **kw)
+    d = _render_ref(oracle, tris, [0, 0, 1, 1], [floor, blocker], [lit(False)], _camera_down(3.0), W, H, **kw)
+    floor_px = np.all(np.isclose(d, b), axis=-1)            # pixels that see the floor, lit regardless
+    assert floor_px.mean() > 0.3
+    assert (c[floor_px].sum(-1) < d[floor_px].sum(-1)).any()  # some of them are shadowed by the sheet
+
+
+# ---------------------------------------------------------------- Lenses::SphericalSegmentLense
+def test_spherical_segment_lens_rays_closed_form(oracle):
+    """Camera rays of the spherical segment lens against a float64 statement of
+    SphericalSegmentLense::getPixelSamples + Ray >>= Transform: angles alpha * screen position,
+    target on the focal sphere, origin on the asin(u)*sensor disc (point map by the rows),
+    direction through the axes."""
+    W, H, fx, fy = 40, 30, 2, 3
+    cam = S.Camera.look_at((1.0, 2.0, -3.0), (0.2, 0.5, 4.0), fov_deg=50.0)
+    rows = np.array([0.9, 0.1, -0.2, 0.05, 1.1, 0.3, 0.2, -0.25, 0.95], np.float32)   # any point map
+    lp = PL.spherical_segment_lens((70.0, 40.0), 25.0, 0.6, (1.0, 0.5, 0.25))
+    t = oracle.Octree(_floor(), 16, 12)
+    R = oracle.Renderer(t, [0, 0], [PL.lambert((1, 1, 1))], [], cam.as_array(), W, H, (fx, fy), 1, lens=2,
+                        lens_params=lp, cam_rows=rows)
+    c = cam.as_array().astype(np.float64)
+    rng = np.random.default_rng(5)
+    for _ in range(60):
+        px, py, i, j = rng.integers(0, W), rng.integers(0, H), rng.integers(0, fx), rng.integers(0, fy)
+        u = rng.uniform(0, 1, 4).astype(np.float32)
+        o, d = oracle.camera_ray(R, px, py, i, j, u)
+        ps = 1.0 / H
+        sx = (px - W / 2.0) * ps + (i + 0.5) * ps / fx
+        sy = (H / 2.0 - py) * ps + (j + 0.5) * ps / fy
+        jx, jy = (u[0] * 2 - 1) * ps / fx / 2, (u[1] * 2 - 1) * ps / fy / 2
+        h, v = math.radians((sx + jx) * 70.0), math.radians((sy + jy) * 40.0)
+        target = np.array([math.sin(h) * math.cos(v), math.sin(v), math.cos(h) * math.cos(v)]) * 25.0
+        a = 2 * math.pi * float(u[2])
+        org = np.array([math.cos(a), math.sin(a), 0.0]) * math.asin(float(u[3])) * 0.6
+        dl = (target - org) / np.linalg.norm(target - org)
+        want_o = rows.astype(np.float64).reshape(3, 3) @ org + c[0:3]
+        want_d = dl[0] * c[3:6] + dl[1] * c[6:9] + dl[2] * c[9:12]
+        assert np.allclose(o, want_o, atol=2e-5) and np.allclose(d, want_d, atol=2e-5), (o, want_o, d, want_d)
+
+
+def test_spherical_segment_lens_filter_scales_the_image(oracle):
+    sc = S.spheres_scene(4, seed=2)
+    kw = dict(lens=2, cam_rows=None)
+    t = oracle.Octree(sc.tris, 16, 12)
+    mk = lambda f: oracle.Renderer(t, sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(), 24, 16, (1, 1), 2,
+                                   lens_params=PL.spherical_segment_lens(60.0, 50.0, 0.1, f), **kw)
+    a = mk((1.0, 1.0, 1.0)).render(nthreads=4)
+    b = mk((0.5, 0.25, 2.0)).render(nthreads=4)
+    assert a.max() > 0
+    np.testing.assert_allclose(b, a * np.array([0.5, 0.25, 2.0], np.float32), rtol=1e-6)

@@ -40,7 +40,10 @@ MIXED = [lambda a: PL.dumb_based_shader((0.9, 0.7, 0.4), 16.0, a, 0.5), lambda a
 LIGHTS = [PL.directional_light((1.5, 1.4, 1.2), (0.3, -1.0, 0.2), 40.0),
           PL.soft_directional_light((1.0, 1.0, 1.0), (-0.2, -1.0, 0.4), 40.0, 0.05),
           PL.spherical_light((40.0, 40.0, 40.0), (1.0, 6.0, -1.0), 0.7),
-          PL.area_light((-0.5, 5.0, -0.5), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (10.0, 10.0, 10.0))]
+          PL.area_light((-0.5, 5.0, -0.5), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), (10.0, 10.0, 10.0)),
+          PL.spot_light((1.0, 0.9, 0.8), 60.0, (1.0, 6.0, -1.0), (-0.1, -1.0, 0.1), 25.0, 50.0, 1.5, 0.3, samples=3),
+          PL.spot_light((0.8, 0.9, 1.0), 40.0, (-2.0, 5.0, 1.0), (0.3, -1.0, -0.2), 10.0, 70.0, 0.7, 0.5,
+                        emit_from_sphere=True, cast_shadows=False, samples=2)]
 
 
 def _pair(oracle, sc, W, H, spp, depth, normals=None, lens="stochastic", **kw):
@@ -149,3 +152,19 @@ def test_adjoint_and_batches_reject_reference_plugins(dev):
         g.backward(torch.zeros_like(f.data))
     with pytest.raises(D.DumbRayError):
         D.RenderBatch([(g, f, 1, 0)], tuning={"tile_blocks": 1, "groups_per_cu": 1, "refill_min": 1})
+
+
+@pytest.mark.parametrize("plugins", ["benchmark", "reference"])
+def test_spherical_segment_lens_matches_oracle(dev, oracle, plugins):
+    sc = S.spheres_scene(6, seed=4)
+    if plugins == "reference":
+        sc.materials = [PL.dumb_based_shader((0.5, 0.5, 0.5), 8.0, m["albedo"], 0.2) for m in sc.materials]
+        sc.lights = LIGHTS[4:6]
+    lp = PL.spherical_segment_lens((70.0, 45.0), 12.0, 0.3, (1.0, 0.8, 0.6))
+    rows = np.array(sc.camera.as_array()[3:12], np.float32) * np.float32(1.25)      # a scaled point map
+    print(f"[case] spherical segment lens ({plugins} plugins) 40x30 spp 2x2 depth 3", flush=True)
+    g, o = _pair(oracle, sc, 40, 30, (2, 2), 3, lens="spherical_segment", lens_params=lp, cam_rows=rows)
+    img = _gpu(g)
+    ref = o.render(nthreads=8) if plugins == "benchmark" else o.render_ref(nthreads=8)
+    assert ref.max() > 0
+    _close(img, ref)

@@ -258,6 +258,8 @@ def _compare(ctx, sf):
     assert same(ctx.light_table, PL.light_table(sf.lights)[: len(sf.lights)])
     assert ctx.max_bounces == sf.max_bounces and ctx.samples_per_pixel == sf.fsaa
     assert ctx.boxing == ["height", "width", "min", "max"][sf.boxing] and ctx.ignore_backfaces == sf.ignore_backfaces
+    assert ctx.lens == sf.lens and same(ctx.lens_params, np.array(sf.lens_params, np.float32))
+    assert same(ctx.cam_rows, sf.cam_rows)
 
 
 def test_example_scene_fixture_is_the_reference_file():
@@ -290,7 +292,11 @@ FULL_SCENE = """
   ],
   "lights": [
     {"type": "simple_soft_directional", "direction": [-1, -2, 0.5], "color": [0.5, 0.25, 0.125], "softness": 0.3},
-    {"type": "simple_spherical", "color": [100, 90, 80], "position": [-3, 9, 2.5], "radius": 1.25}
+    {"type": "simple_spherical", "color": [100, 90, 80], "position": [-3, 9, 2.5], "radius": 1.25},
+    {"type": "spotlight", "color": [1, 0.9, 0.8], "luminosity": 40, "position": [2, 6, -1], "target": [0, 0, 0],
+     "hotspot_angle": 25, "falloff_angle": 50, "falloff_power": 1.5, "emitter_size": 0.4, "emitter_shape": "sphere",
+     "cast_shadows": false, "sample_count": 3},
+    {"type": "spotlight", "direction": [0, -1, 0.2], "falloff_angle": -10, "emitter_shape": "oval", "sample_count": 99}
   ],
   "groups": [
     {"name": "pair", "elements": [
@@ -356,7 +362,7 @@ def test_full_scene_matches_oracle(tmp_path):
     _compare(rest, sf_rest)
     assert same(ctx.tris[n_all:], sf.tris[n_all:]) and np.array_equal(ctx.mat_id, sf.mat_id)
     assert ctx.boxing == "width" and ctx.max_bounces == 3 and ctx.samples_per_pixel == (2, 3)
-    assert not ctx.ignore_backfaces and ctx.light_table.shape[0] == 3 and ctx.material_table.shape[0] == 7
+    assert not ctx.ignore_backfaces and ctx.light_table.shape[0] == 5 and ctx.material_table.shape[0] == 7
 
 
 BAD_SCENES = [
@@ -364,7 +370,8 @@ BAD_SCENES = [
     '{"materials": [{"type": "unknown"}]}',
     '{"materials": [{"type": "dumb_based", "diffuse_texture": "wood"}]}',
     '{"materials": [{"type": "dumb_based", "specular": "high"}]}',
-    '{"lights": [{"type": "spotlight"}]}',
+    '{"lights": [{"type": "spotlight", "luminosity": "bright"}]}',
+    '{"lights": [{"type": "spotlight", "cast_shadows": 1}]}',
     '{"lights": [{"type": "simple_spherical", "radius": [1]}]}',
     '{"objects": [{"mesh": {"obj": "mesh.obj", "object": "A"}}]}',                         # no material
     '{"objects": [{"mesh": {"obj": "mesh.obj", "object": "Z"}, "material": {"type": "fresnel_color"}}]}',
@@ -372,7 +379,9 @@ BAD_SCENES = [
     '{"objects": [{"mesh": {"primitive": "sphere"}, "material": {"type": "fresnel_color"}}]}',
     '{"objects": ["nogroup"]}',
     '{"objects": [{"elements": [{"mesh": {"obj": "mesh.obj", "object": "A"}}]}]}',          # unresolved material
-    '{"camera": {"lense": {"type": "spherical_segment"}}}',
+    '{"camera": {"lense": {"type": "spherical_segment", "sensor": "big"}}}',
+    '{"camera": {"lense": {"type": "spherical_segment", "filter": 3}}}',
+    '{"camera": {"lense": {"type": "fisheye"}}}',
     '{"camera": {"lense": {"angle": 30}}}',
     '{"renderer": {"pixel": {"boxing": "diagonal"}}}',
     '{"renderer": {"pixel": {"samples_per_pixel": [1]}}}',
@@ -408,3 +417,25 @@ def test_reference_scene_with_primitives_is_rejected():
         SF.load(path)
     with pytest.raises(D.DumbRayError):
         D.DumbRenderContext.fromFile(path)
+
+
+def test_spherical_segment_lens_settings(tmp_path):
+    text = ('{"camera": {"lense": {"type": "spherical_segment", "angle": 75, "distance": 40, "sensor": 0.5, '
+            '"filter": [1, 0.5, 0.25]}, "transform": {"position": [1, 2, 3], "rotation": [10, 20, 30], '
+            '"scale": [1, 2, 1]}}}')
+    ctx = D.DumbRenderContext.fromText(text)
+    sf = SF.parse(text)
+    _compare(ctx, sf)
+    assert ctx.lens == PL.SPHERICAL_SEGMENT_LENS
+    assert ctx.lens_params.tolist() == [75.0, 75.0, 40.0, 0.5, 1.0, 0.5, 0.25, 0.0]
+    # rows carry the scale, the direction axes divide it out again
+    np.testing.assert_allclose(ctx.cam_rows.reshape(3, 3)[:, 1] / 2.0, ctx.camera[3:12].reshape(3, 3)[:, 1], rtol=1e-6)
+    # defaults of the lens
+    d = D.DumbRenderContext.fromText('{"camera": {"lense": {"type": "spherical_segment"}}}')
+    assert d.lens_params.tolist() == [60.0, 60.0, 128.0, 1.0, 1.0, 1.0, 1.0, 0.0]
+
+
+def test_oracle_asin(oracle):
+    for x in np.linspace(0.0, 0.999999, 400):
+        assert abs(oracle.asin_f(float(np.float32(x))) - math.asin(float(np.float32(x)))) < 3e-7
+    assert oracle.asin_f(0.0) == 0.0 and abs(oracle.asin_f(1.0) - math.pi / 2) < 2e-7

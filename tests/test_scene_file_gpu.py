@@ -38,7 +38,8 @@ def _pair(path, W, H, iterations=1, **over):
     boxing = {"height": 0, "width": 1, "min": 2, "max": 3}[over.get("boxing", ctx.boxing)]
     o = oracle.Renderer(oracle.Octree(sf.tris, 8, 24), sf.mat_id, sf.materials, sf.lights, sf.cam, W, H,
                         over.get("samples_per_pixel", sf.fsaa), over.get("max_bounces", sf.max_bounces),
-                        ignore_backfaces=sf.ignore_backfaces, normals=sf.normals, boxing=boxing)
+                        ignore_backfaces=sf.ignore_backfaces, normals=sf.normals, boxing=boxing, lens=sf.lens,
+                        lens_params=sf.lens_params, cam_rows=sf.cam_rows)
     frame = None
     for it in range(1, iterations + 1):
         print(f"[case] {os.path.basename(path)} {W}x{H} iteration {it} {over}", flush=True)
@@ -115,3 +116,19 @@ def test_grouped_scene_matches_oracle(dev, tmp_path):
         (tmp_path / name).write_text(textwrap.dedent(text))
     g, _ = _pair(str(tmp_path / "scene.dumb"), 40, 32, iterations=2)
     assert g.boxing == 3 and g.fsaa_x == 2 and not g.ignore_backfaces
+
+
+def test_example_scene_spherical_segment_lens_matches_oracle(dev, tmp_path):
+    """The example scene seen through the spherical segment lens (angles, aperture, filter)."""
+    import re
+
+    with open(EXAMPLE) as f:
+        text = f.read()
+    text = text.replace('"files/cargo.obj"', '"%s"' % os.path.join(os.path.dirname(EXAMPLE), "files", "cargo.obj"))
+    text = re.sub(r'"lense"\s*:\s*\{[^}]*\}',
+                  '"lense": {"type": "spherical_segment", "angle": 40, "distance": 140, "sensor": 2.5, '
+                  '"filter": [1.0, 0.9, 0.7]}', text)
+    path = tmp_path / "lens.dumb"
+    path.write_text(text)
+    g, _ = _pair(str(path), 64, 40)
+    assert g.lens == D.plugins.SPHERICAL_SEGMENT_LENS
