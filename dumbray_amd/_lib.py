@@ -37,6 +37,7 @@ def _declare(L):
         "dr_cast_stats": (c_int, [vp, vp, c_int, c_uint, vp, vp, vp, vp, vp]),
         "dr_scene_set_materials": (c_int, [vp, vp, c_int]),
         "dr_scene_set_normals": (c_int, [vp, vp, c_int]),
+        "dr_scene_set_texcoords": (c_int, [vp, vp, c_int]),
         "dr_render_blocks": (c_int, [vp, vp, vp, c_int, c_int, vp]),
         "dr_backward": (c_int, [vp, vp, vp, vp, vp, vp]),
         "dr_batch_create": (c_int, [vp, c_int, vp, vp]),
@@ -49,8 +50,13 @@ def _declare(L):
         "dr_scene_file_geometry": (c_int, [vp, vp, vp, vp]),
         "dr_scene_file_tables": (c_int, [vp, vp, vp]),
         "dr_scene_file_destroy": (None, [vp]),
+        "dr_scene_file_texture_info": (c_int, [vp, vp, vp, vp]),
+        "dr_scene_file_textures": (c_int, [vp, vp, vp, vp]),
+        "dr_scene_file_texcoords": (c_int, [vp, vp]),
         "dr_frame_to_rgba8": (c_int, [vp, c_int, c_int, vp, vp]),
         "dr_png_write": (c_int, [c_char_p, vp, c_int, c_int]),
+        "dr_png_info": (c_int, [c_char_p, vp, vp]),
+        "dr_png_read": (c_int, [c_char_p, vp, c_int, c_int]),
     }
     for name, (res, args) in decl.items():
         f = getattr(L, name)

@@ -122,7 +122,7 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
   s->max_depth = flat.max_depth;
   s->ntris = (int)t->host.tris.size();
   s->shade_host.resize(s->ntris);
-  s->faces_host.resize((size_t)s->ntris * dr::kFaceWords);
+  s->faces_host.assign((size_t)s->ntris * dr::kFaceWords, 0.0f);   // texture coordinates start at 0
   for (int i = 0; i < s->ntris; i++) {
     const dr::Tri &tr = t->host.tris[i];
     const dr::V3 n = dr::cross(tr.b - tr.a, tr.c - tr.a);
@@ -149,6 +149,16 @@ int dr_scene_set_normals(dr_scene s, const float *normals, int ntris) {
       const dr::V3 n = dr::normalized(dr::cross(c - a, b - a));
       for (int k = 0; k < 3; k++) { F[9 + 3 * k] = n.x; F[10 + 3 * k] = n.y; F[11 + 3 * k] = n.z; }
     }
+  }
+  dr::dev_free(s->faces);
+  return dr::dev_upload(s->faces, s->faces_host.data(), s->faces_host.size() * sizeof(float));
+}
+
+int dr_scene_set_texcoords(dr_scene s, const float *uv, int ntris) {
+  if (!s || ntris != s->ntris) return dr::fail(DR_ERR_ARG, "dr_scene_set_texcoords: one coordinate triple per triangle required");
+  for (int i = 0; i < ntris; i++) {
+    float *F = &s->faces_host[(size_t)i * dr::kFaceWords];
+    for (int k = 0; k < 6; k++) F[18 + k] = uv ? uv[6 * (size_t)i + k] : 0.0f;
   }
   dr::dev_free(s->faces);
   return dr::dev_upload(s->faces, s->faces_host.data(), s->faces_host.size() * sizeof(float));

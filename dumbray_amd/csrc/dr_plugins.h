@@ -1,7 +1,7 @@
 // The reference's own render plugins on the device (DR_PLUGINS_REFERENCE): shaders
-// DumbBasedShader, SimpleStochasticShader, FresnelShader and DefaultShaderGeneric
-// (without textures: every ColoredTexture is its colour), lights SimpleDirectionalLight,
-// SimpleSoftDirectionalLight and SphericalLightEmitter.  Each function keeps the
+// DumbBasedShader, SimpleStochasticShader, FresnelShader and DefaultShaderGeneric with
+// their textures (ColoredTexture colour, diffuse, normal-map and alpha-cutout maps), lights
+// SimpleDirectionalLight, SimpleSoftDirectionalLight, SphericalLightEmitter and Spotlight.  Each function keeps the
 // reference's expression order, so a host evaluation of the same code gives the
 // same bits.
 //
@@ -17,7 +17,7 @@ namespace dr {
 constexpr float kPiRef = 3.14159265359f;   // PI, Vector3.h:8
 constexpr int kSpecTries = 224;            // tries of the specular lobe's rejection loop
 constexpr int kMaxTransmit = 16;           // surfaces a light photon may pass through
-constexpr int kFaceWords = 18;             // per triangle: a, b, c, then the three vertex normals
+constexpr int kFaceWords = 24;             // per triangle: a, b, c, the three vertex normals, then u, v of a, b, c
 
 DR_HD bool is_reference_kind(int kind) { return kind >= DR_DUMB_BASED; }
 DR_HD V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
@@ -113,13 +113,72 @@ DR_HD V3 get_masses(V3 a, V3 b, V3 c, V3 center) {
   return divf(v3(aa, bb, cc), aa + bb + cc);
 }
 
+// ---------------------------------------------------------------- textures
+// Textures of the reference shaders: the tables of dr_render_params.
+struct TexCtx {
+  const dr_material_maps *maps;
+  const dr_texture *tex;
+  const float4 *texels;
+};
+DR_HD TexCtx tex_ctx(const dr_render_params &P) { return TexCtx{P.maps, P.textures, (const float4 *)P.texels}; }
+// The material's maps, or nullptr (dr_material.reserved[0] = 1 + index).
+DR_HD const dr_material_maps *material_maps(const TexCtx &T, const dr_material &M) {
+  return (M.reserved[0] > 0 && T.maps) ? &T.maps[M.reserved[0] - 1] : nullptr;
+}
+
+// Texture::operator()(Vector2) (Texture.impl.cuh:78-90): wrap to [0,1) through the integer
+// casts, flip y, then the nearest texel of the 2x2 cell (FILTER_NONE) or bilinear weights.
+DR_HD float4 texture_sample(const dr_texture &T, const float4 *__restrict__ texels, float px, float py) {
+  float x = (px + ((float)(((int)(-px)) + 2)));
+  x -= ((float)((int)x));
+  float y = (py + ((float)(((int)(-py)) + 2)));
+  y -= ((float)((int)y));
+  y = (1.0f - y);
+  const uint32_t w = (uint32_t)T.width, h = (uint32_t)T.height;
+  const float posX = (x * ((float)w)), posY = (y * ((float)h));
+  const uint32_t minX = (((uint32_t)posX) % w), minY = (((uint32_t)posY) % h);
+  const uint32_t maxX = ((minX + 1) % w), maxY = ((minY + 1) % h);
+  const float offX = (posX - ((float)minX)), offY = (posY - ((float)minY));
+  const float4 *D = texels + T.offset;
+  if (T.filter == 0) return D[((offY <= 0.5f) ? minY : maxY) * w + ((offX <= 0.5f) ? minX : maxX)];
+  const float4 c00 = D[minY * w + minX], c01 = D[maxY * w + minX], c10 = D[minY * w + maxX], c11 = D[maxY * w + maxX];
+  const float w00 = ((1.0f - offX) * (1.0f - offY)), w01 = ((1.0f - offX) * offY);
+  const float w10 = (offX * (1.0f - offY)), w11 = (offX * offY);
+  float4 r;
+  r.x = (((c00.x * w00) + (c01.x * w01)) + (c10.x * w10)) + (c11.x * w11);
+  r.y = (((c00.y * w00) + (c01.y * w01)) + (c10.y * w10)) + (c11.y * w11);
+  r.z = (((c00.z * w00) + (c01.z * w01)) + (c10.z * w10)) + (c11.z * w11);
+  r.w = (((c00.w * w00) + (c01.w * w01)) + (c10.w * w10)) + (c11.w * w11);
+  return r;
+}
+// The texture sample of a map at texture coordinate (u, v): coord = (uv ^ tiling) + offset
+// (ColoredTexture::operator(), DumbStructs.cuh:87-91).
+DR_HD float4 map_sample(const TexCtx &T, const dr_texture_map &m, float u, float v) {
+  return texture_sample(T.tex[m.texture], T.texels, (u * m.tiling[0]) + m.offset[0], (v * m.tiling[1]) + m.offset[1]);
+}
+// A ColoredTexture's colour at (u, v): the colour, times the texture sample when it has one.
+DR_HD void map_color(const TexCtx &T, const dr_texture_map *m, const float col[3], float u, float v, float out[3]) {
+  if (m == nullptr || m->texture < 0) {
+    for (int c = 0; c < 3; c++) out[c] = col[c];
+    return;
+  }
+  const float4 t = map_sample(T, *m, u, v);
+  out[0] = col[0] * t.x;
+  out[1] = col[1] * t.y;
+  out[2] = col[2] * t.z;
+}
+
 // A surface point as the reference shaders see it.
 struct RefHit {
   V3 p;
   V3 ng;   // Triangle::normal = (c - a) x (b - a) (Triangle.impl.h:233-235)
   V3 nI;   // BakedTriFace::norm.massCenter(getMasses(p)) (Triangle.impl.h:318-320)
+  V3 ns;   // ColoredTexture::getNormal of normalColor (DumbStructs.cuh:107-131): nI normalised,
+           // or turned by the normal map
+  float tu, tv;   // BakedTriFace::tex.massCenter(getMasses(p))
 };
-__device__ __forceinline__ RefHit ref_hit(const float *__restrict__ faces, int h, V3 p) {
+__device__ __forceinline__ RefHit ref_hit(const float *__restrict__ faces, int h, V3 p, const TexCtx &T,
+                                          const dr_material_maps *mp) {
   const float *F = faces + (size_t)kFaceWords * h;
   const V3 a = v3(F[0], F[1], F[2]), b = v3(F[3], F[4], F[5]), c = v3(F[6], F[7], F[8]);
   RefHit H;
@@ -128,29 +187,63 @@ __device__ __forceinline__ RefHit ref_hit(const float *__restrict__ faces, int h
   const V3 m = get_masses(a, b, c, p);
   const V3 na = v3(F[9], F[10], F[11]), nb = v3(F[12], F[13], F[14]), nc = v3(F[15], F[16], F[17]);
   H.nI = (na * m.x + nb * m.y) + nc * m.z;
+  H.ns = normalized(H.nI);
+  H.tu = (F[18] * m.x + F[20] * m.y) + F[22] * m.z;
+  H.tv = (F[19] * m.x + F[21] * m.y) + F[23] * m.z;
+  if (mp != nullptr && mp->map[2].texture >= 0) {
+    // tangent from the texture coordinates (DumbStructs.cuh:111-127)
+    float du, dv;
+    V3 vd;
+    const V3 pm = (a * m.x + b * m.y) + c * m.z;
+    if (m.x < 0.999f) { du = F[18] - H.tu; dv = F[19] - H.tv; vd = a - pm; }
+    else { du = F[20] - H.tu; dv = F[21] - H.tv; vd = b - pm; }
+    const V3 n = H.ns;
+    const V3 vco = normalized(cross(vd, n));
+    vd = cross(vco, n);
+    const V3 u = normalized((vd * dv) - (vco * du));
+    const V3 v = cross(u, n);
+    const float4 rgb = map_sample(T, mp->map[2], H.tu, H.tv);
+    const float mx = (mp->normal_color[1] * rgb.z);
+    const float my = (mp->normal_color[0] * ((rgb.x * 2.0f) - 1.0f));
+    const float mz = (mp->normal_color[2] * ((rgb.y * 2.0f) - 1.0f));
+    H.ns = normalized(((n * mx) + (u * my)) + (v * mz));
+  }
   return H;
+}
+// DumbBasedShader's alpha cutout (DumbBasedShader.impl.cuh:29, :68): the cutout texture's
+// alpha at the hit is at most the threshold.
+DR_HD bool alpha_cut(const TexCtx &T, const dr_material_maps *mp, const RefHit &H) {
+  if (mp == nullptr || mp->map[3].texture < 0) return false;
+  return mp->alpha_threshold >= map_sample(T, mp->map[3], H.tu, H.tv).w;
 }
 
 // getReflectedColor of a reference shader for a photon (direction pd, colour pc) toward
 // the observer direction ob; direct = PHOTON_TYPE_DIRECT_ILLUMINATION.
 DR_HD void reflect_ref(const dr_material &M, const RefHit &H, V3 pd, const float pc[3], V3 ob, bool direct, int stype,
-                       float out[3]) {
+                       const TexCtx &T, float out[3]) {
+  const dr_material_maps *mp = material_maps(T, M);
   if (M.kind == DR_DUMB_BASED) {                      // DumbBasedShader.impl.cuh:63-109
+    if (alpha_cut(T, mp, H)) {
+      const bool through = dot(normalized(ob), normalized(pd)) >= 0.9999f;
+      for (int c = 0; c < 3; c++) out[c] = through ? pc[c] : 0.0f;
+      return;
+    }
     if (dot(normalized(H.ng), pd) > 0.0f && !direct) {
       for (int c = 0; c < 3; c++) out[c] = pc[c];
       return;
     }
     if (dot(H.nI, ob) < 0.0f) { out[0] = out[1] = out[2] = 0.0f; return; }
     const float spec = M.p[0], metal = M.p[1];
-    const V3 n = normalized(H.nI);
+    const V3 n = H.ns;
     const V3 wi = neg(normalized(pd));
     const bool brdf_m = direct || stype == 1;
     float brdf[3] = {0.0f, 0.0f, 0.0f};
     if (brdf_m) {
       const V3 w0 = normalized(ob);
       const V3 wh = normalized(w0 + wi);
-      float fwi[3];
-      fresnel(M.albedo, wh, wi, fwi);
+      float fcol[3], fwi[3];
+      map_color(T, mp ? &mp->map[0] : nullptr, M.albedo, H.tu, H.tv, fcol);
+      fresnel(fcol, wh, wi, fwi);
       float mult = 1.0f;
       if (direct) {
         const float nwh = dot(n, wh);
@@ -164,8 +257,10 @@ DR_HD void reflect_ref(const dr_material &M, const RefHit &H, V3 pd, const float
     const bool diff_m = direct || stype == 2;
     float dif[3] = {0.0f, 0.0f, 0.0f};
     if (diff_m) {
+      float dcol[3];
+      map_color(T, mp ? &mp->map[1] : nullptr, M.diffuse, H.tu, H.tv, dcol);
       const float nwi = dot(n, wi);
-      for (int c = 0; c < 3; c++) dif[c] = M.diffuse[c] * nwi;
+      for (int c = 0; c < 3; c++) dif[c] = dcol[c] * nwi;
     }
     float col[3];
     if (brdf_m && diff_m) {
@@ -187,8 +282,9 @@ DR_HD void reflect_ref(const dr_material &M, const RefHit &H, V3 pd, const float
     for (int c = 0; c < 3; c++) out[c] = (pc[c] * M.albedo[c]) * s;
   } else if (M.kind == DR_FRESNEL) {                  // FresnelShader.impl.cuh:14-19
     const V3 light = neg(normalized(pd));
-    float f[3];
-    fresnel(M.albedo, H.nI, light, f);
+    float fcol[3], f[3];
+    map_color(T, mp ? &mp->map[0] : nullptr, M.albedo, H.tu, H.tv, fcol);
+    fresnel(fcol, H.nI, light, f);
     for (int c = 0; c < 3; c++) out[c] = f[c] * pc[c];
   } else {                                            // DefaultShaderGeneric::cast, DefaultShader.impl.cuh:15-33
     const V3 cam = (H.p + ob) - H.p;
@@ -204,9 +300,11 @@ DR_HD void reflect_ref(const dr_material &M, const RefHit &H, V3 pd, const float
 DR_HD bool sample_ref(const dr_render_params &P, const dr_material &M, const RefHit &H, V3 d, uint32_t pix,
                       uint32_t samp, int layer, const float u[4], V3 &dir, int &type) {
   if (M.kind == DR_DUMB_BASED) {                      // DumbBasedShader.impl.cuh:18-61
+    const TexCtx T = tex_ctx(P);
+    if (alpha_cut(T, material_maps(T, M), H)) { dir = d; type = 3; return true; }   // cut out: straight through
     V3 normal = normalized(H.ng);
     if (u[3] <= M.p[1]) {
-      const V3 n = normalized(H.nI);
+      const V3 n = H.ns;
       if (dot(n, d) >= 0.0f) return false;
       const V3 r = normalized(reflection(d, n));
       if (dot(r, normal) < 0.0f) normal = normal * -1.0f;

@@ -210,7 +210,8 @@ struct RefPath {
     const V3 du = normalized(din);
     nu = facing_normal(v3(sh.x, sh.y, sh.z), du);
     H.p = w.h.p;
-    if (is_reference_kind(M.kind)) H = ref_hit(tb.faces, h, w.h.p);
+    const TexCtx T = tex_ctx(P);
+    if (is_reference_kind(M.kind)) H = ref_hit(tb.faces, h, w.h.p, T, material_maps(T, M));
     has = false;
     stype = 0;
     float sw = 1.0f;
@@ -232,7 +233,7 @@ struct RefPath {
       float F[3];
       if (is_reference_kind(M.kind)) {
         const float white[3] = {1.0f, 1.0f, 1.0f};
-        reflect_ref(M, H, neg(sdir), white, neg(din), false, stype, F);
+        reflect_ref(M, H, neg(sdir), white, neg(din), false, stype, T, F);
       } else if (M.kind == DR_METAL) {
         const float f5 = schlick5(nu, sdir);
         for (int c = 0; c < 3; c++) F[c] = M.albedo[c] + (1.0f - M.albedo[c]) * f5;
@@ -256,7 +257,7 @@ struct RefPath {
       }
     } else {
       float f[3];
-      reflect_ref(M, H, pd, pc, neg(din), true, 0, f);
+      reflect_ref(M, H, pd, pc, neg(din), true, 0, tex_ctx(P), f);
       for (int c = 0; c < 3; c++) L[c] += beta[c] * f[c];
     }
   }
@@ -273,9 +274,10 @@ struct RefPath {
         const int hb = w.h.id;
         const dr_material Mb = P.materials[__float_as_int(tb.shade[hb].w)];
         if (is_reference_kind(Mb.kind) && trans < kMaxTransmit) {
-          const RefHit B = ref_hit(tb.faces, hb, w.h.p);
+          const TexCtx T = tex_ctx(P);
+          const RefHit B = ref_hit(tb.faces, hb, w.h.p, T, material_maps(T, Mb));
           float tc[3];
-          reflect_ref(Mb, B, pd, pc, pd, true, 0, tc);
+          reflect_ref(Mb, B, pd, pc, pd, true, 0, T, tc);
           if (((tc[0] + tc[1]) + tc[2]) > kVecEps) {
             for (int c = 0; c < 3; c++) pc[c] = tc[c];
             o = w.h.p + normalized(pd) * (64.0f * kVecEps);
@@ -656,6 +658,8 @@ static int check_params(dr_scene s, const dr_render_params *p) {
     return fail(DR_ERR_ARG, "render: unknown lens");
   if (p->plugins != DR_PLUGINS_BENCHMARK && p->plugins != DR_PLUGINS_REFERENCE)
     return fail(DR_ERR_ARG, "render: unknown plugin set");
+  if (p->nmaps < 0 || p->ntextures < 0 || (p->nmaps > 0 && !p->maps) || (p->ntextures > 0 && (!p->textures || !p->texels)))
+    return fail(DR_ERR_ARG, "render: texture tables missing");
   return DR_OK;
 }
 

@@ -2,7 +2,7 @@
 // Wavefront OBJ meshes it names, turned into the host arrays the rest of the
 // library takes (dr_octree_build triangles, dr_scene_set_normals and
 // dr_scene_set_materials inputs, dr_material / dr_light tables, and the camera
-// and settings of dr_render_params).
+// and settings of dr_render_params, plus textures and texture coordinates).
 //
 // Behaviour follows DumbRenderContext.cu (fromFile :310-335, fromDson :337-371,
 // groups / items / sub-groups :106-271, camera :476-514, renderer :515-640,
@@ -29,6 +29,8 @@
 
 namespace dr {
 int fail(int code, const std::string &msg);
+bool png_decode_file(const std::string &path, std::vector<unsigned char> &rgba, uint32_t &w, uint32_t &h,
+                     std::string &err);
 }
 
 namespace {
@@ -296,7 +298,7 @@ struct Corner {
   int v, t, n;
 };
 struct ObjMesh {
-  std::vector<std::vector<F3>> face_pos, face_nrm;   // per face, per corner
+  std::vector<std::vector<F3>> face_pos, face_nrm, face_tex;   // per face, per corner
 };
 
 class ObjReader {
@@ -353,7 +355,7 @@ class ObjReader {
       ObjMesh m;
       std::unordered_map<int, F3> nmap;
       for (int i = ob.first; i < ob.second; i++) {
-        std::vector<F3> P, N;
+        std::vector<F3> P, N, T;
         for (const Corner &k : faces[i]) {
           if (k.v < 0 || k.v >= nv) { err = "vertex index out of range"; return false; }
           auto it = nmap.find(k.n);
@@ -364,9 +366,11 @@ class ObjReader {
           if (k.t >= (int)texs.size()) { err = "texture index out of range"; return false; }
           P.push_back(verts[k.v]);
           N.push_back(it->second);
+          T.push_back(k.t >= 0 ? texs[k.t] : f3(0.0f, 0.0f, 0.0f));   // a corner without one: Vertex::zero()
         }
         m.face_pos.push_back(std::move(P));
         m.face_nrm.push_back(std::move(N));
+        m.face_tex.push_back(std::move(T));
       }
       meshes.push_back(std::move(m));
     }
@@ -436,7 +440,7 @@ class ObjReader {
 
 // ------------------------------------------------------------------ scene context
 struct Unit {
-  F3 v[3], n[3];
+  F3 v[3], n[3], t[3];
   int mat;
 };
 
@@ -481,6 +485,10 @@ struct dr_scene_file_s {
   std::unordered_map<std::string, std::unordered_map<std::string, ObjMesh>> obj_files;
   std::map<std::string, Group> groups;
   std::vector<dr_material> materials;
+  std::vector<dr_material_maps> maps;
+  std::vector<dr_texture> textures;
+  std::vector<float> texels;
+  std::unordered_map<std::string, int> texture_names;
   std::vector<dr_light> lights;
   std::vector<Unit> geometry;
   Transform camera;
@@ -609,21 +617,84 @@ struct dr_scene_file_s {
     return true;
   }
 
-  // ---------------------------------------------------------------- plugins
-  // ColoredTexture::fromDson (DumbStructs.cu:7-37) without textures: a texture entry
-  // that names an image is rejected; tiling / offset must still be vectors.
-  bool colored(const DValue &d, const char *ck, const char *tk, const char *tl, const char *of, F3 &color) {
-    if (d.has(ck) && !vec_from(d.at(ck), color, err)) return false;
-    if (d.has(tk)) {
-      const DValue &t = d.at(tk);
-      if (t.kind == DValue::STRING || t.kind == DValue::DICT)
-        return error(std::string("Error: textures are not supported (") + tk + ")");
-      if (t.kind == DValue::ARRAY) return error("Error: Unsupported dson type for texture");
+  // ---------------------------------------------------------------- textures
+  // DumbRenderContext::getImageId (DumbRenderContext.cu:374-452): bool / null / number = no
+  // texture; a string names a texture ("name::" first, then "png::"); a dict loads its "png"
+  // file (next to the scene file when that path opens) as a new texture, with optional
+  // "filtering" (none / [bilinear]) and "name".
+  bool image_id(const DValue &v, int &id) {
+    if (v.kind == DValue::BOOL || v.kind == DValue::NUL || v.kind == DValue::NUMBER) { id = -1; return true; }
+    if (v.kind == DValue::STRING) {
+      auto it = texture_names.find("name::" + v.str);
+      if (it == texture_names.end()) it = texture_names.find("png::" + v.str);
+      if (it == texture_names.end()) return error("Error: Texture \"" + v.str + "\" not found");
+      id = it->second;
+      return true;
     }
-    F3 dummy{};
-    if (d.has(tl) && !vec_from(d.at(tl), dummy, err)) return false;
-    if (d.has(of) && !vec_from(d.at(of), dummy, err)) return false;
+    if (v.kind != DValue::DICT) return error("Error: Unsupported dson type for texture");
+    if (!v.has("png")) return error("Error: Image dict incomplete");
+    if (v.at("png").kind != DValue::STRING) return error("Error: Image 'png' entry MUST BE a string");
+    const std::string &name = v.at("png").str;
+    std::string path = source_path + name;
+    if (!open(path)) path = name;
+    std::vector<unsigned char> rgba;
+    uint32_t w = 0, h = 0;
+    std::string perr;
+    if (!dr::png_decode_file(path, rgba, w, h, perr)) return error("Error: Could not read file: \"" + path + "\"");
+    id = (int)textures.size();
+    dr_texture t{};
+    t.offset = (int32_t)(texels.size() / 4);
+    t.width = (int32_t)w;
+    t.height = (int32_t)h;
+    t.filter = 1;                                   // Texture's default FILTER_BILINEAR
+    for (unsigned char b : rgba) texels.push_back(((float)b) / 255.0f);   // Images::getTexturePNG
+    textures.push_back(t);
+    texture_names["png::" + path] = id;
+    texture_names["png::" + name] = id;
+    if (v.has("filtering")) {
+      const DValue &f = v.at("filtering");
+      if (f.kind != DValue::STRING) return error("Error: Image 'filtering' entry MUST BE a string");
+      if (f.str == "none") textures[id].filter = 0;
+      else if (f.str == "bilinear") textures[id].filter = 1;
+      else return error("Error: Image filter can be only \"none\"/[\"bilinear\"] (got: \"" + f.str + "\")");
+    }
+    if (v.has("name")) {
+      if (v.at("name").kind != DValue::STRING) return error("Error: Image name entry MUST BE a string");
+      texture_names["name::" + v.at("name").str] = id;
+    }
     return true;
+  }
+
+  // ---------------------------------------------------------------- plugins
+  // ColoredTexture::fromDson (DumbStructs.cu:7-37): colour, texture, tiling and offset keys.
+  bool colored(const DValue &d, const char *ck, const char *tk, const char *tl, const char *of, F3 &color,
+               dr_texture_map &map) {
+    if (d.has(ck) && !vec_from(d.at(ck), color, err)) return false;
+    if (d.has(tk) && !image_id(d.at(tk), map.texture)) return false;
+    F3 v{};
+    if (d.has(tl)) {
+      if (!vec_from(d.at(tl), v, err)) return false;
+      map.tiling[0] = v.x; map.tiling[1] = v.y;
+    }
+    if (d.has(of)) {
+      if (!vec_from(d.at(of), v, err)) return false;
+      map.offset[0] = v.x; map.offset[1] = v.y;
+    }
+    return true;
+  }
+  static dr_texture_map no_texture() {
+    dr_texture_map m{};
+    m.texture = -1;
+    m.tiling[0] = m.tiling[1] = 1.0f;
+    return m;
+  }
+  // Keeps the material's maps when any of them has a texture (dr_material.reserved[0]).
+  void attach_maps(dr_material &out, const dr_material_maps &mp) {
+    bool any = false;
+    for (int k = 0; k < 4; k++) any = any || mp.map[k].texture >= 0;
+    if (!any) return;
+    maps.push_back(mp);
+    out.reserved[0] = (int32_t)maps.size();
   }
   bool number_of(const DValue &d, const char *k, float &out, const char *msg) {
     if (!d.has(k)) return true;
@@ -652,10 +723,14 @@ struct dr_scene_file_s {
     float metal = s.metal, specular = s.spec;
     F3 diffuse = scale(s.diffuse, 1.0f / ((metal == 1.0f) ? 1.0f : (1.0f - metal)));   // Color /= f
     F3 normal_color = f3(1.0f, 1.0f, 1.0f), alpha = f3(1.0f, 1.0f, 1.0f);
-    if (!colored(d, "fresnel", "fresnel_texture", "fresnel_tiling", "fresnel_offset", fresnel)) return false;
-    if (!colored(d, "diffuse", "diffuse_texture", "diffuse_tiling", "diffuse_offset", diffuse)) return false;
-    if (!colored(d, "normal_color", "normal_texture", "normal_tiling", "normal_offset", normal_color)) return false;
-    if (!colored(d, "alpha_cutout_color", "alpha_cutout_texture", "alpha_cutout_tiling", "alpha_cutout_offset", alpha))
+    dr_material_maps mp{};
+    for (int k = 0; k < 4; k++) mp.map[k] = no_texture();
+    if (!colored(d, "fresnel", "fresnel_texture", "fresnel_tiling", "fresnel_offset", fresnel, mp.map[0])) return false;
+    if (!colored(d, "diffuse", "diffuse_texture", "diffuse_tiling", "diffuse_offset", diffuse, mp.map[1])) return false;
+    if (!colored(d, "normal_color", "normal_texture", "normal_tiling", "normal_offset", normal_color, mp.map[2]))
+      return false;
+    if (!colored(d, "alpha_cutout_color", "alpha_cutout_texture", "alpha_cutout_tiling", "alpha_cutout_offset", alpha,
+                 mp.map[3]))
       return false;
     float thresh = 0.5f;
     if (!number_of(d, "alpha_cutout_threshold", thresh, "Error: DumbBasedShader alpha_cutout_threshold value hat to be a number..."))
@@ -667,6 +742,9 @@ struct dr_scene_file_s {
     out.albedo[0] = s.fresnel.x; out.albedo[1] = s.fresnel.y; out.albedo[2] = s.fresnel.z;
     out.diffuse[0] = s.diffuse.x; out.diffuse[1] = s.diffuse.y; out.diffuse[2] = s.diffuse.z;
     out.p[0] = s.spec; out.p[1] = s.metal;
+    mp.normal_color[0] = normal_color.x; mp.normal_color[1] = normal_color.y; mp.normal_color[2] = normal_color.z;
+    mp.alpha_threshold = thresh;
+    attach_maps(out, mp);
     return true;
   }
 
@@ -704,9 +782,14 @@ struct dr_scene_file_s {
       else if (c == "aluminum") color = f3(0.91f, 0.92f, 0.92f);
       else if (c == "silver") color = f3(0.95f, 0.93f, 0.88f);
     }
-    if (!colored(d, "color", "texture", "tiling", "offset", color)) return false;
+    dr_material_maps mp{};
+    for (int k = 0; k < 4; k++) mp.map[k] = no_texture();
+    if (!colored(d, "color", "texture", "tiling", "offset", color, mp.map[0])) return false;
     out.kind = DR_FRESNEL;
     out.albedo[0] = color.x; out.albedo[1] = color.y; out.albedo[2] = color.z;
+    mp.normal_color[0] = mp.normal_color[1] = mp.normal_color[2] = 1.0f;
+    mp.alpha_threshold = 0.5f;
+    attach_maps(out, mp);
     return true;
   }
 
@@ -944,9 +1027,11 @@ struct dr_scene_file_s {
       for (size_t cur = 2; cur < P.size(); cur++) {
         Unit u;
         const size_t idx[3] = {0, (size_t)prev, cur};
+        const auto &TC = m.face_tex[f];
         for (int k = 0; k < 3; k++) {
           u.v[k] = it.t.point(P[idx[k]]);
           u.n[k] = it.t.normal(N[idx[k]]);
+          u.t[k] = TC[idx[k]];   // texture coordinates are not transformed (BakedTriMesh.impl.h:16-18)
         }
         u.mat = it.mat;
         out.push_back(u);
@@ -1183,6 +1268,32 @@ int dr_scene_file_geometry(dr_scene_file f, float *tris, float *normals, int32_t
     }
     if (mat_id) mat_id[i] = u.mat;
   }
+  return DR_OK;
+}
+
+int dr_scene_file_texture_info(dr_scene_file f, int *ntextures, int64_t *ntexels, int *nmaps) {
+  if (!f) return dr::fail(DR_ERR_ARG, "dr_scene_file_texture_info: null scene file");
+  if (ntextures) *ntextures = (int)f->textures.size();
+  if (ntexels) *ntexels = (int64_t)(f->texels.size() / 4);
+  if (nmaps) *nmaps = (int)f->maps.size();
+  return DR_OK;
+}
+
+int dr_scene_file_textures(dr_scene_file f, dr_texture *textures, float *texels, dr_material_maps *maps) {
+  if (!f) return dr::fail(DR_ERR_ARG, "dr_scene_file_textures: null scene file");
+  if (textures && !f->textures.empty()) std::memcpy(textures, f->textures.data(), f->textures.size() * sizeof(dr_texture));
+  if (texels && !f->texels.empty()) std::memcpy(texels, f->texels.data(), f->texels.size() * sizeof(float));
+  if (maps && !f->maps.empty()) std::memcpy(maps, f->maps.data(), f->maps.size() * sizeof(dr_material_maps));
+  return DR_OK;
+}
+
+int dr_scene_file_texcoords(dr_scene_file f, float *uv) {
+  if (!f || !uv) return dr::fail(DR_ERR_ARG, "dr_scene_file_texcoords: bad arguments");
+  for (size_t i = 0; i < f->geometry.size(); i++)
+    for (int k = 0; k < 3; k++) {
+      uv[6 * i + 2 * k] = f->geometry[i].t[k].x;
+      uv[6 * i + 2 * k + 1] = f->geometry[i].t[k].y;
+    }
   return DR_OK;
 }
 

@@ -133,6 +133,18 @@ class DeviceScene:
         check(lib().dr_scene_set_normals(self._h, n.ctypes.data_as(ctypes.c_void_p), self.ntris),
               "dr_scene_set_normals")
 
+    def set_texcoords(self, uv=None):
+        """Texture coordinates of every triangle (BakedTriFace::tex): (ntris, 6) float32 (u, v of
+        a, b, c) or None for zeros."""
+        if uv is None:
+            check(lib().dr_scene_set_texcoords(self._h, None, self.ntris), "dr_scene_set_texcoords")
+            return
+        t = np.ascontiguousarray(np.asarray(uv, np.float32).reshape(-1, 6))
+        if t.shape[0] != self.ntris:
+            raise DumbRayError("set_texcoords: one row of 6 floats per triangle required")
+        check(lib().dr_scene_set_texcoords(self._h, t.ctypes.data_as(ctypes.c_void_p), self.ntris),
+              "dr_scene_set_texcoords")
+
     def cast(self, rays, clip_backfaces: bool = True, exclude=None, points: bool = False, stream=None):
         """Batch closest hit.  rays: torch float32 (n, 6) on the GPU.  Returns dict of
         torch tensors id (int32, -1 = miss), t (float32, FLT_MAX on miss) [, point]."""

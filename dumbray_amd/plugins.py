@@ -16,7 +16,9 @@ does, in fp32, so the table holds the same values the reference object would:
   lenses      STOCHASTIC_LENS (SimpleStochasticLense), PERSPECTIVE_LENS (DefaultPerspectiveLense),
               SPHERICAL_SEGMENT_LENS (spherical_segment_lens: SphericalSegmentLense.impl.cuh:6-34)
 
-Textures are not supported: every ColoredTexture is its colour.
+Textures: ``texture`` / ``texture_map`` / ``with_maps`` describe a ColoredTexture's texture
+part (colour, diffuse, normal and alpha-cutout maps of DumbBasedShader, FresnelShader's colour);
+``maps_table`` / ``texture_tables`` give the device tables.
 """
 from __future__ import annotations
 
@@ -187,15 +189,85 @@ def is_reference_light(l) -> bool:
     return LIGHT_KINDS[l.get("kind", "quad")] != 0
 
 
+# ---------------------------------------------------------------- textures
+MAP_SLOTS = {"color": 0, "diffuse": 1, "normal": 2, "alpha": 3}
+
+
+def texture(rgba, filtering="bilinear"):
+    """A Texture (Texture.cuh:13-60): rgba = (H, W, 4) float32 texels, row 0 at the top, or
+    (H, W, 4) uint8 bytes converted as Images::getTexturePNG does (byte / 255.0f, Images.cu:45-53)."""
+    a = np.asarray(rgba)
+    if a.dtype == np.uint8:
+        a = a.astype(np.float32) / f32(255.0)
+    a = np.ascontiguousarray(a, np.float32)
+    assert a.ndim == 3 and a.shape[2] == 4 and a.shape[0] > 0 and a.shape[1] > 0
+    return {"rgba": a, "filter": {"none": 0, "bilinear": 1}[filtering]}
+
+
+def texture_map(index, tiling=(1.0, 1.0), offset=(0.0, 0.0)):
+    """The texture part of a ColoredTexture: texture `index` of the scene's texture list."""
+    return {"texture": int(index), "tiling": tuple(float(f32(t)) for t in tiling),
+            "offset": tuple(float(f32(o)) for o in offset)}
+
+
+def with_maps(material, color=None, diffuse=None, normal=None, alpha=None, normal_color=(1.0, 1.0, 1.0),
+              alpha_threshold=0.5):
+    """The material with textures: color (DumbBased fresnelColor / Fresnel colour), diffuse,
+    normal (normal map, channel weights normal_color) and alpha (cutout below alpha_threshold)."""
+    m = dict(material)
+    m["maps"] = {"color": color, "diffuse": diffuse, "normal": normal, "alpha": alpha,
+                 "normal_color": _rgb(normal_color), "alpha_threshold": float(f32(alpha_threshold))}
+    return m
+
+
+def maps_table(materials) -> np.ndarray:
+    """(n, 24) float32 rows in dr_material_maps layout, one per material with maps, in order."""
+    rows = []
+    for m in materials:
+        mp = m.get("maps")
+        if mp is None:
+            continue
+        r = np.zeros(24, np.float32)
+        for slot, k in MAP_SLOTS.items():
+            tm = mp.get(slot)
+            r[5 * k] = np.array([-1 if tm is None else tm["texture"]], np.int32).view(np.float32)[0]
+            r[5 * k + 1:5 * k + 3] = 1.0
+            if tm is not None:
+                r[5 * k + 1:5 * k + 3] = tm["tiling"]
+                r[5 * k + 3:5 * k + 5] = tm["offset"]
+        r[20:23] = mp["normal_color"]
+        r[23] = mp["alpha_threshold"]
+        rows.append(r)
+    return np.stack(rows) if rows else np.zeros((0, 24), np.float32)
+
+
+def texture_tables(textures):
+    """(descriptors (n, 4) int32 in dr_texture layout, texels (total, 4) float32)."""
+    desc = np.zeros((len(textures), 4), np.int32)
+    chunks, off = [], 0
+    for i, t in enumerate(textures):
+        h, w = t["rgba"].shape[:2]
+        desc[i] = (off, w, h, t["filter"])
+        chunks.append(t["rgba"].reshape(-1, 4))
+        off += w * h
+    texels = np.concatenate(chunks) if chunks else np.zeros((0, 4), np.float32)
+    return desc, np.ascontiguousarray(texels, np.float32)
+
+
 # ---------------------------------------------------------------- device tables
 def material_table(materials) -> np.ndarray:
-    """(n, 12) float32 rows in dr_material layout."""
+    """(n, 12) float32 rows in dr_material layout; reserved[0] = 1 + the material's row in
+    maps_table for materials with textures."""
     t = np.zeros((max(len(materials), 1), 12), np.float32)
+    k = 0
     for i, m in enumerate(materials):
         t[i, 0] = np.array([MATERIAL_KINDS[m["kind"]]], np.int32).view(np.float32)[0]
         t[i, 1:4] = m.get("albedo", (0.0, 0.0, 0.0))
         t[i, 4:7] = m.get("diffuse", (0.0, 0.0, 0.0))
         t[i, 7:10] = m.get("p", (0.0, 0.0, 0.0))
+        if m.get("maps") is not None:
+            k += 1
+            t[i, 10] = np.array([k], np.int32).view(np.float32)[0]
     return t
 
 

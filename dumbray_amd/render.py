@@ -37,7 +37,8 @@ class RenderParams(ctypes.Structure):
                 ("cam", ctypes.c_float * 13), ("materials", ctypes.c_void_p), ("lights", ctypes.c_void_p),
                 ("nmat", ctypes.c_int32), ("nlights", ctypes.c_int32), ("lens", ctypes.c_int32),
                 ("plugins", ctypes.c_int32), ("boxing", ctypes.c_int32), ("lens_p", ctypes.c_float * 8),
-                ("cam_rows", ctypes.c_float * 9)]
+                ("cam_rows", ctypes.c_float * 9), ("maps", ctypes.c_void_p), ("textures", ctypes.c_void_p),
+                ("texels", ctypes.c_void_p), ("nmaps", ctypes.c_int32), ("ntextures", ctypes.c_int32)]
 
 
 BOXING = {"height": 0, "width": 1, "min": 2, "max": 3}   # DumbRenderer::BoxingMode (DumbRenderer.cuh:12-18)
@@ -132,7 +133,8 @@ class DumbRenderer:
     def __init__(self, scene: DeviceScene, mat_id, materials, lights, camera, width: int, height: int,
                  samples_per_pixel=(1, 1), max_bounces: int = 2, rr_start: int = 1 << 30, rr_survive: float = 1.0,
                  ignore_backfaces: bool = True, seed: int = 0x5EED, device="cuda", lens: str = "stochastic",
-                 normals=None, boxing="height", lens_params=None, cam_rows=None):
+                 normals=None, boxing="height", lens_params=None, cam_rows=None, textures=None, uv=None,
+                 maps=None):
         import torch
 
         self.scene = scene
@@ -157,6 +159,24 @@ class DumbRenderer:
         mt, lt, self.nmat, self.nlights, self.plugins = _tables(materials, lights)
         self.material_table = torch.from_numpy(mt.copy()).to(device)
         self.light_table = torch.from_numpy(lt.copy()).to(device)
+        # textures: a list of plugins.texture dicts or (descriptors, texels) tables; maps from the
+        # material dicts, or given as a dr_material_maps table with material tables
+        if maps is None:
+            maps = PL.maps_table(materials) if not isinstance(materials, np.ndarray) else np.zeros((0, 24), np.float32)
+        maps = np.ascontiguousarray(maps, np.float32).reshape(-1, 24)
+        if textures is None:
+            desc, texels = np.zeros((0, 4), np.int32), np.zeros((0, 4), np.float32)
+        elif isinstance(textures, tuple):
+            desc, texels = (np.ascontiguousarray(textures[0], np.int32).reshape(-1, 4),
+                            np.ascontiguousarray(textures[1], np.float32).reshape(-1, 4))
+        else:
+            desc, texels = PL.texture_tables(textures)
+        self.nmaps, self.ntextures = int(maps.shape[0]), int(desc.shape[0])
+        self.maps_table = torch.from_numpy(maps.copy() if maps.size else np.zeros((1, 24), np.float32)).to(device)
+        self.texture_table = torch.from_numpy(desc.copy() if desc.size else np.zeros((1, 4), np.int32)).to(device)
+        self.texels = torch.from_numpy(texels.copy() if texels.size else np.zeros((1, 4), np.float32)).to(device)
+        if uv is not None:
+            scene.set_texcoords(uv)
 
     # ------------------------------------------------ BufferedRenderer / Renderer iteration interface
     # (Renderer.cu:136-156, BufferedRenderer.cu:7): iterate() renders iteration k + 1 into the
@@ -220,6 +240,8 @@ class DumbRenderer:
         p.lights = self.light_table.data_ptr()
         p.nmat, p.nlights = self.nmat, self.nlights
         p.lens, p.plugins, p.boxing = self.lens, self.plugins, self.boxing
+        p.maps, p.nmaps = self.maps_table.data_ptr(), self.nmaps
+        p.textures, p.texels, p.ntextures = self.texture_table.data_ptr(), self.texels.data_ptr(), self.ntextures
         return p
 
     def renderBlocksGPU(self, frame: BlockBasedFrameBuffer, start_block: int, end_block: int, iteration: int = 1,

@@ -3,9 +3,10 @@
 ``DumbRenderContext`` mirrors the reference's DumbRenderContext (DumbRenderContext.cuh,
 DumbRenderContext.cu:295-371): ``buildFromFile`` / ``fromFile`` / ``fromText`` load a
 scene through the library's loader (dr_scene_file_*, C++), giving the triangles in push
-order, their vertex normals (BakedTriFace::norm) and material ids, the material and
-light tables, the camera and the renderer settings (bounces, samples per pixel, boxing,
-ignore backfaces).  ``octree()`` builds Octree<Renderable<BakedTriFace>> with the
+order, their vertex normals and texture coordinates (BakedTriFace::norm, ::tex) and
+material ids, the material, material-map and light tables, the textures of the scene's
+PNG files, the camera and the renderer settings (bounces, samples per pixel, boxing,
+ignore backfaces, lens).  ``octree()`` builds Octree<Renderable<BakedTriFace>> with the
 reference's constants (split threshold 8, maximum depth 24, Octree.cuh:11-13) and
 ``renderer(width, height)`` a DumbRenderer with the file's settings; the window size
 that picks the resolution in the reference is the caller's choice here.
@@ -40,6 +41,10 @@ class DumbRenderContext:
         self.ignore_backfaces, self.lens = True, 0
         self.lens_params = np.zeros(8, np.float32)
         self.cam_rows = np.zeros(9, np.float32)
+        self.uv = np.zeros((0, 6), np.float32)
+        self.texture_desc = np.zeros((0, 4), np.int32)
+        self.texels = np.zeros((0, 4), np.float32)
+        self.maps = np.zeros((0, 24), np.float32)
 
     # ------------------------------------------------------------ loading
     @classmethod
@@ -82,6 +87,17 @@ class DumbRenderContext:
             c.lens = int(st.lens)
             c.lens_params = np.array(st.lens_p[:], np.float32)
             c.cam_rows = np.array(st.cam_rows[:], np.float32)
+            c.uv = np.zeros((n.value, 6), np.float32)
+            if n.value:
+                check(L.dr_scene_file_texcoords(h, c.uv.ctypes.data), "dr_scene_file_texcoords")
+            nt, ntx, nm = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int()
+            check(L.dr_scene_file_texture_info(h, ctypes.byref(nt), ctypes.byref(ntx), ctypes.byref(nm)),
+                  "dr_scene_file_texture_info")
+            c.texture_desc = np.zeros((nt.value, 4), np.int32)
+            c.texels = np.zeros((ntx.value, 4), np.float32)
+            c.maps = np.zeros((nm.value, 24), np.float32)
+            check(L.dr_scene_file_textures(h, c.texture_desc.ctypes.data, c.texels.ctypes.data, c.maps.ctypes.data),
+                  "dr_scene_file_textures")
             return c
         finally:
             L.dr_scene_file_destroy(h)
@@ -108,7 +124,8 @@ class DumbRenderContext:
         tree = tree if tree is not None else self.octree()
         kw = dict(samples_per_pixel=self.samples_per_pixel, max_bounces=self.max_bounces,
                   ignore_backfaces=self.ignore_backfaces, lens=self.lens, boxing=self.boxing,
-                  lens_params=self.lens_params, cam_rows=self.cam_rows)
+                  lens_params=self.lens_params, cam_rows=self.cam_rows, maps=self.maps,
+                  textures=(self.texture_desc, self.texels), uv=self.uv)
         kw.update(overrides)
         return DumbRenderer(tree.upload(), self.mat_id, self.material_table, self.light_table, self.camera, width,
                             height, seed=seed, device=device, normals=self.normals, **kw)

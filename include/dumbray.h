@@ -91,13 +91,38 @@ enum {
   DR_DEFAULT_SHADER = 6       /* DefaultShaderGeneric, DefaultShader.impl.cuh:5-52 */
 };
 
+/* A texture (Texture, Texture.cuh:13-60): width x height RGBA texels, row 0 at the top,
+ * starting at texel `offset` of the texel array; filter 0 = FILTER_NONE, 1 = FILTER_BILINEAR. */
+typedef struct {
+  int32_t offset, width, height, filter;
+} dr_texture;
+
+/* A ColoredTexture's texture part (DumbStructs.cuh:70-92): the texture (-1: the colour
+ * alone), sampled at texture coordinate * tiling + offset. */
+typedef struct {
+  int32_t texture;
+  float tiling[2], offset[2];
+} dr_texture_map;
+
+/* Textures of one material (96 B), referenced by dr_material.reserved[0] = 1 + index:
+ *   map[0] the colour's texture (DumbBasedShader fresnelColor, FresnelShader color),
+ *   map[1] DumbBasedShader diffuseColor, map[2] normalColor, map[3] alphaCutout;
+ * normal_color = normalColor's colour (the normal map's channel weights), alpha_threshold =
+ * alphaCutoutThreshold (DumbBasedShader.cuh:25-31). */
+typedef struct {
+  dr_texture_map map[4];
+  float normal_color[3];
+  float alpha_threshold;
+} dr_material_maps;
+
 /* One material (48 B).  Fields by kind:
  *   LAMBERT / MIRROR / METAL: albedo = Lambert albedo / mirror reflectance / Schlick F0.
  *   DUMB_BASED: albedo = fresnel colour, diffuse = diffuse colour already scaled by
  *     (1 - metal) as the constructor does, p[0] = specular exponent, p[1] = metal in [0, 1].
  *   SIMPLE_STOCHASTIC / DEFAULT_SHADER: albedo = colour, p[0] = diffuse, p[1] = gloss
  *     (both divided by their sum when it exceeds 1, as the constructors do), p[2] = shininess.
- *   FRESNEL: albedo = fresnel colour. */
+ *   FRESNEL: albedo = fresnel colour.
+ * reserved[0]: 0, or 1 + the index of the material's dr_material_maps (DUMB_BASED, FRESNEL). */
 typedef struct {
   int32_t kind;
   float albedo[3];
@@ -180,6 +205,10 @@ typedef struct {
                                      sensor size, filter r, g, b, unused */
   float cam_rows[9];              /* the camera Transform's rows trans.x, trans.y, trans.z (scale applied): the
                                      point map of lens origins (Transform.impl.h:85-87); used by that lens */
+  const dr_material_maps *maps;   /* device table [nmaps] (materials with textures), or NULL */
+  const dr_texture *textures;     /* device table [ntextures], or NULL */
+  const float *texels;            /* device RGBA fp32 texels of every texture */
+  int32_t nmaps, ntextures;
 } dr_render_params;
 
 /* Attach per-triangle material ids (host int32[ntris], ntris = the octree's triangle
@@ -191,6 +220,11 @@ int dr_scene_set_materials(dr_scene s, const int32_t *mat_id, int ntris);
  * (c - a) x (b - a) of each triangle (Triangle::normal, Triangle.impl.h:233-235), which
  * is also the default.  Used by the reference shaders (DR_PLUGINS_REFERENCE). */
 int dr_scene_set_normals(dr_scene s, const float *normals, int ntris);
+
+/* Texture coordinates of every triangle (BakedTriFace::tex, BakedTriMesh.h:8): host
+ * float[ntris * 6] (u, v of a, b, c), or NULL for zeros (the default).  Used by textured
+ * materials of the reference shaders. */
+int dr_scene_set_texcoords(dr_scene s, const float *uv, int ntris);
 
 /* DumbRenderer::renderBlocksGPU (DumbRenderer.cu:87-113, override of
  * BlockRenderer::renderBlocksGPU, BlockRenderer.cuh:43-44): renders framebuffer
@@ -252,7 +286,7 @@ void dr_batch_destroy(dr_batch b);
  * (MeshReader.impl.h) and PolyMesh::bake (PolyMesh.impl.h:140-152).  Host memory only.
  * Supported plugin types: materials dumb_based, simple_stochastic, fresnel_color;
  * lights simple_soft_directional, simple_spherical, spotlight; lenses simple_stochastic,
- * spherical_segment.  Textures and primitive meshes are rejected with an error.  Thread, device and block settings are checked and ignored. */
+ * spherical_segment; PNG textures.  Primitive meshes are rejected with an error.  Thread, device and block settings are checked and ignored. */
 typedef struct dr_scene_file_s *dr_scene_file;
 
 /* Renderer settings and camera of a scene file (DumbRenderContextData::RendererSettings,
@@ -283,6 +317,13 @@ int dr_scene_file_info(dr_scene_file f, dr_scene_file_settings *settings, int *n
 int dr_scene_file_geometry(dr_scene_file f, float *tris, float *normals, int32_t *mat_id);
 /* Material and light tables (materials[nmat], lights[nlights]); either may be NULL. */
 int dr_scene_file_tables(dr_scene_file f, dr_material *materials, dr_light *lights);
+/* Textures loaded from the scene's PNG files (DumbRenderContext::getImageId,
+ * DumbRenderContext.cu:374-452) and the materials' maps: counts, then the tables
+ * (textures[ntextures], texels[ntexels * 4] RGBA = byte / 255, maps[nmaps]); any pointer may be NULL. */
+int dr_scene_file_texture_info(dr_scene_file f, int *ntextures, int64_t *ntexels, int *nmaps);
+int dr_scene_file_textures(dr_scene_file f, dr_texture *textures, float *texels, dr_material_maps *maps);
+/* Texture coordinates of every triangle for dr_scene_set_texcoords (uv[ntris * 6]). */
+int dr_scene_file_texcoords(dr_scene_file f, float *uv);
 void dr_scene_file_destroy(dr_scene_file f);
 
 /* ---------------------------------------------------------------- image output */
@@ -295,6 +336,11 @@ int dr_frame_to_rgba8(const float *frame, int width, int height, uint8_t *out, v
 /* lodepng::encode as Images::saveBufferPNG calls it (Images.cu:33): an 8-bit RGBA PNG
  * file of a host row-major RGBA8 image. */
 int dr_png_write(const char *path, const uint8_t *rgba, int width, int height);
+/* lodepng::decode as Images::getTexturePNG calls it (Images.cu:38-42): a PNG file (any colour
+ * type and bit depth, tRNS, interlaced or not) as 8-bit RGBA, row 0 at the top.
+ * dr_png_info gives the size; dr_png_read fills rgba[width*height*4] (host). */
+int dr_png_info(const char *path, int *width, int *height);
+int dr_png_read(const char *path, uint8_t *rgba, int width, int height);
 
 /* dr_cast plus per-ray work counters of the traversal that ran (measurement only;
  * no counterpart in the reference, which prints timings only): out_counts is a

@@ -79,6 +79,20 @@ class OraLight(ctypes.Structure):
                 ("radius", ctypes.c_float), ("reserved", ctypes.c_int32)]
 
 
+class OraTextures(ctypes.Structure):
+    _fields_ = [("uv", ctypes.c_void_p), ("maps", ctypes.c_void_p), ("nmaps", ctypes.c_int), ("tex", ctypes.c_void_p),
+                ("ntex", ctypes.c_int), ("texels", ctypes.c_void_p)]
+
+
+def texture_sample(desc, texels, x, y):
+    """Texture::operator()(Vector2) of texture `desc` (offset, w, h, filter) over texels (n, 4)."""
+    d = np.ascontiguousarray(desc, np.int32).reshape(4)
+    t = np.ascontiguousarray(texels, np.float32).reshape(-1, 4)
+    out = np.zeros(4, np.float32)
+    lib().ora_texture_sample(d.ctypes.data, t.ctypes.data, ctypes.c_float(x), ctypes.c_float(y), out.ctypes.data)
+    return out
+
+
 class OraRenderCfg(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("fsaa_x", ctypes.c_int32),
                 ("fsaa_y", ctypes.c_int32), ("max_bounces", ctypes.c_int32), ("rr_start", ctypes.c_int32),
@@ -93,15 +107,51 @@ KINDS = {"lambert": 0, "mirror": 1, "metal": 2, "dumb_based": 3, "simple_stochas
 LIGHT_KINDS = {"quad": 0, "area": 0, "directional": 1, "soft_directional": 2, "spherical": 3, "spot": 4}
 
 
-def pack_material(m, out=None):
-    """OraMaterial from a material dict: kind, albedo and the optional diffuse / p fields."""
+def pack_material(m, out=None, maps_row=0):
+    """OraMaterial from a material dict: kind, albedo and the optional diffuse / p fields;
+    maps_row = 1 + the material's row of the maps table (0: none)."""
     r = out if out is not None else OraMaterial()
     r.kind = KINDS[m["kind"]]
+    r.reserved[0] = int(maps_row)
     for c in range(3):
         r.albedo[c] = float(np.float32(m.get("albedo", (0, 0, 0))[c]))
         r.diffuse[c] = float(np.float32(m.get("diffuse", (0, 0, 0))[c]))
         r.p[c] = float(np.float32(m.get("p", (0, 0, 0))[c]))
     return r
+
+
+def pack_maps(materials):
+    """Rows of ora_material_maps (24 floats: 4 maps of texture, tiling, offset; normal colour;
+    alpha threshold) of the materials that have "maps", in order."""
+    slots = ("color", "diffuse", "normal", "alpha")
+    rows = []
+    for m in materials:
+        mp = m.get("maps")
+        if mp is None:
+            continue
+        r = np.zeros(24, np.float32)
+        for k, name in enumerate(slots):
+            tm = mp.get(name)
+            r[5 * k:5 * k + 1] = np.array([-1 if tm is None else tm["texture"]], np.int32).view(np.float32)
+            r[5 * k + 1:5 * k + 3] = 1.0                     # a map without texture: tiling 1, offset 0
+            if tm is not None:
+                r[5 * k + 1:5 * k + 5] = list(tm["tiling"]) + list(tm["offset"])
+        r[20:23] = mp["normal_color"]
+        r[23] = mp["alpha_threshold"]
+        rows.append(r)
+    return np.stack(rows) if rows else np.zeros((0, 24), np.float32)
+
+
+def pack_textures(textures):
+    """(descriptors (n,4) int32: texel offset, width, height, filter; texels (m,4) float32)."""
+    desc, chunks, off = [], [], 0
+    for t in textures:
+        h, w = t["rgba"].shape[:2]
+        desc.append((off, w, h, t["filter"]))
+        chunks.append(np.asarray(t["rgba"], np.float32).reshape(-1, 4))
+        off += w * h
+    return (np.array(desc, np.int32).reshape(-1, 4),
+            np.concatenate(chunks) if chunks else np.zeros((0, 4), np.float32))
 
 
 def pack_light(l, out=None):
@@ -129,6 +179,10 @@ def _bind_render(L):
     L.ora_render_ref.restype = ctypes.c_int
     L.ora_render_ref.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int]
+    L.ora_render_ref_tex.restype = ctypes.c_int
+    L.ora_render_ref_tex.argtypes = [vp, vp, vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
+    L.ora_texture_sample.argtypes = [vp, vp, ctypes.c_float, ctypes.c_float, vp]
     L.ora_philox.argtypes = [vp, vp, vp]
     L.ora_sincos_2pi.argtypes = [ctypes.c_float, vp, vp]
     L.ora_pow.restype = ctypes.c_float
@@ -241,18 +295,45 @@ class Renderer:
 
     def __init__(self, tree, mat_id, materials, lights, cam_array, width, height, samples_per_pixel=(1, 1),
                  max_bounces=2, rr_start=1 << 30, rr_survive=1.0, ignore_backfaces=True, seed=0x5EED, lens=0,
-                 normals=None, boxing=0, lens_params=None, cam_rows=None):
+                 normals=None, boxing=0, lens_params=None, cam_rows=None, textures=None, uv=None, maps=None):
         self.tree = tree
         self.mat_id = np.ascontiguousarray(mat_id, np.int32)
         self.mats = (OraMaterial * max(len(materials), 1))()
+        k = 0
         for i, m in enumerate(materials):
-            pack_material(m, self.mats[i])
+            row = 0
+            if m.get("maps") is not None or m.get("maps_row"):
+                k += 1
+                row = m.get("maps_row", k)
+            pack_material(m, self.mats[i], row)
         self.nmat = len(materials)
         self.lights = (OraLight * max(len(lights), 1))()
         for i, l in enumerate(lights):
             pack_light(l, self.lights[i])
         self.nlights = len(lights)
         self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1)
+        # textures: texture tables (descriptors (n,4) int32, texels (m,4) float32) or None; maps:
+        # dr_material_maps rows (k,24) float32 (the material's reserved[0] = 1 + row); uv (ntris,6)
+        self._tx = None
+        if maps is None and any(m.get("maps") is not None for m in materials):
+            maps = pack_maps(materials)
+        if textures is not None or uv is not None or maps is not None:
+            self._uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1)
+            self._maps = np.ascontiguousarray(np.zeros((0, 24)) if maps is None else maps, np.float32).reshape(-1, 24)
+            if textures is None:
+                desc, texels = np.zeros((0, 4)), np.zeros((0, 4))
+            elif isinstance(textures, tuple):
+                desc, texels = textures
+            else:
+                desc, texels = pack_textures(textures)
+            self._desc = np.ascontiguousarray(desc, np.int32).reshape(-1, 4)
+            self._texels = np.ascontiguousarray(texels, np.float32).reshape(-1, 4)
+            tx = OraTextures()
+            tx.uv = None if self._uv is None else self._uv.ctypes.data
+            tx.maps, tx.nmaps = (self._maps.ctypes.data if self._maps.size else None), self._maps.shape[0]
+            tx.tex, tx.ntex = (self._desc.ctypes.data if self._desc.size else None), self._desc.shape[0]
+            tx.texels = self._texels.ctypes.data if self._texels.size else None
+            self._tx = tx
         cfg = OraRenderCfg()
         cfg.width, cfg.height = int(width), int(height)
         cfg.fsaa_x, cfg.fsaa_y = int(samples_per_pixel[0]), int(samples_per_pixel[1])
@@ -302,9 +383,10 @@ class Renderer:
         r0, r1 = (0, H) if rows is None else (int(rows[0]), int(rows[1]))
         out = np.zeros((r1 - r0, W, 3), np.float32)
         nrm = None if self.normals is None else self.normals.ctypes.data
-        rc = lib().ora_render_ref(self.tree.h, self.mat_id.ctypes.data, ctypes.addressof(self.mats), self.nmat,
-                                  ctypes.addressof(self.lights), self.nlights, nrm, ctypes.addressof(self.cfg),
-                                  out.ctypes.data, r0, r1, int(nthreads))
+        tx = None if self._tx is None else ctypes.addressof(self._tx)
+        rc = lib().ora_render_ref_tex(self.tree.h, self.mat_id.ctypes.data, ctypes.addressof(self.mats), self.nmat,
+                                      ctypes.addressof(self.lights), self.nlights, nrm, tx, ctypes.addressof(self.cfg),
+                                      out.ctypes.data, r0, r1, int(nthreads))
         assert rc == 0
         return out
 

@@ -66,6 +66,7 @@ typedef struct {
 	const ora_octree *tree; const int32_t *mat_id; const ora_material *mats; int nmat;
 	const ora_light *lights; int nlights; const ora_render_cfg *cfg;
 	const float *normals;           /* reference estimator: per-vertex normals or NULL */
+	const ora_textures *tx;         /* reference estimator: textures or NULL */
 } rctx;
 
 /* 4 uniforms in [0,1) for (pixel, sample, layer, purpose): purpose 0 = lens,
@@ -323,7 +324,7 @@ int ora_render(const ora_octree *t, const int32_t *mat_id, const ora_material *m
                int nthreads) {
 	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
 	if (row0 < 0 || row1 > cfg->height || row0 > row1) return 1;
-	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, NULL};
+	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, NULL, NULL};
 	return run(&C, out_rgb64 ? 1 : 0, out_rgb, out_rgb64, NULL, NULL, NULL, nthreads, row0, row1);
 }
 
@@ -331,7 +332,7 @@ int ora_backward(const ora_octree *t, const int32_t *mat_id, const ora_material 
                  const ora_light *lights, int nlights, const ora_render_cfg *cfg, const float *dl_dimage,
                  double *g_albedo, double *g_emission, int nthreads) {
 	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
-	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, NULL};
+	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, NULL, NULL};
 	memset(g_albedo, 0, sizeof(double) * nmat * 3);
 	memset(g_emission, 0, sizeof(double) * nlights * 3);
 	return run(&C, 2, NULL, NULL, dl_dimage, g_albedo, g_emission, nthreads, 0, cfg->height);
@@ -342,6 +343,15 @@ int ora_render_ref(const ora_octree *t, const int32_t *mat_id, const ora_materia
                    float *out_rgb, int row0, int row1, int nthreads) {
 	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
 	if (row0 < 0 || row1 > cfg->height || row0 > row1) return 1;
-	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, normals};
+	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, normals, NULL};
+	return run(&C, 3, out_rgb, NULL, NULL, NULL, NULL, nthreads, row0, row1);
+}
+
+int ora_render_ref_tex(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
+                       const ora_light *lights, int nlights, const float *normals, const ora_textures *tx,
+                       const ora_render_cfg *cfg, float *out_rgb, int row0, int row1, int nthreads) {
+	if (nlights > MAX_LIGHTS || cfg->fsaa_x < 1 || cfg->fsaa_y < 1) return 1;
+	if (row0 < 0 || row1 > cfg->height || row0 > row1) return 1;
+	rctx C = {t, mat_id, mats, nmat, lights, nlights, cfg, normals, tx};
 	return run(&C, 3, out_rgb, NULL, NULL, NULL, NULL, nthreads, row0, row1);
 }

@@ -30,6 +30,23 @@ extern "C" {
 enum { ORA_LAMBERT = 0, ORA_MIRROR = 1, ORA_METAL = 2, ORA_DUMB_BASED = 3, ORA_SIMPLE_STOCHASTIC = 4,
        ORA_FRESNEL = 5, ORA_DEFAULT_SHADER = 6 };
 
+/* Same layouts as the library's dr_texture, dr_texture_map and dr_material_maps. */
+typedef struct { int32_t offset, width, height, filter; } ora_texture;
+typedef struct { int32_t texture; float tiling[2], offset[2]; } ora_texture_map;
+typedef struct {
+	ora_texture_map map[4];        /* colour (DumbBased fresnelColor / Fresnel color), diffuse, normal, alpha cutout */
+	float normal_color[3];
+	float alpha_threshold;
+} ora_material_maps;
+/* Textures of a scene: per-triangle texture coordinates (6 floats: u, v of a, b, c), the
+ * materials' maps (ora_material.reserved[0] = 1 + index), texture descriptors and RGBA texels. */
+typedef struct {
+	const float *uv;
+	const ora_material_maps *maps; int nmaps;
+	const ora_texture *tex; int ntex;
+	const float *texels;
+} ora_textures;
+
 /* Same layout as the library's dr_material (48 B). */
 typedef struct {
 	int32_t kind;
@@ -89,6 +106,12 @@ int ora_backward(const ora_octree *t, const int32_t *mat_id, const ora_material 
 int ora_render_ref(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
                    const ora_light *lights, int nlights, const float *normals, const ora_render_cfg *cfg,
                    float *out_rgb, int row0, int row1, int nthreads);
+/* The same with textures (tx may be NULL). */
+int ora_render_ref_tex(const ora_octree *t, const int32_t *mat_id, const ora_material *mats, int nmat,
+                       const ora_light *lights, int nlights, const float *normals, const ora_textures *tx,
+                       const ora_render_cfg *cfg, float *out_rgb, int row0, int row1, int nthreads);
+/* Texture::operator()(Vector2) of one texture (probe for tests): RGBA into out. */
+void ora_texture_sample(const ora_texture *t, const float *texels, float x, float y, float out[4]);
 
 /* Probes of the reference plugins for tests.  photon_type: 0 direct, 1 indirect. */
 float ora_pow(float x, float y);

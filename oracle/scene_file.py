@@ -409,7 +409,7 @@ def read_obj(text):
         if s in ("v", "vn", "vt"):
             x, y, z = T.get_float(), T.get_float(), T.get_float()
             if s == "vt":
-                texs.append((x, y, z))
+                texs.append((x * f32(1), y * f32(1), z * f32(1)))
             else:
                 (verts if s == "v" else norms).append((x * f32(-1), y * f32(1), z * f32(1)))
         elif s == "f":
@@ -460,7 +460,7 @@ def read_obj(text):
                     nidx[n] = norms[n] if n >= 0 else vn[v]
                 if t >= len(texs):
                     raise SceneError("TEXTURE ERROR")
-                face.append((verts[v], nidx[n]))
+                face.append((verts[v], nidx[n], texs[t] if t >= 0 else V(0, 0, 0)))
             mesh.append(face)
         meshes.append(mesh)
     return meshes, names
@@ -494,15 +494,26 @@ _PRESETS["gold"], _PRESETS["glossy"], _PRESETS["matte"] = (_PRESETS["glossy_gold
                                                          _PRESETS["matte_finish"])
 
 
-def _colored(d, ck, tk, tl, of, color):
+def _colored(d, ck, tk, tl, of, color, images=None):
+    """ColoredTexture::fromDson: (colour, map) with map = None or {texture, tiling, offset}."""
     if ck in d:
         color = vector_from(d[ck])
-    if tk in d and (isinstance(d[tk], (str, dict, list))):
-        raise SceneError("textures are not supported")
-    for k in (tl, of):
-        if k in d:
-            vector_from(d[k])
-    return color
+    tex, tiling, offset = -1, (1.0, 1.0), (0.0, 0.0)
+    if tk in d:
+        tex = images.image_id(d[tk]) if images is not None else _no_images(d[tk])
+    if tl in d:
+        v = vector_from(d[tl])
+        tiling = (float(v[0]), float(v[1]))
+    if of in d:
+        v = vector_from(d[of])
+        offset = (float(v[0]), float(v[1]))
+    return color, ({"texture": tex, "tiling": tiling, "offset": offset} if tex >= 0 else None)
+
+
+def _no_images(v):
+    if isinstance(v, (bool, float)) or v is None:
+        return -1
+    raise SceneError("texture outside a scene file")
 
 
 def _number(d, k, cur):
@@ -513,7 +524,7 @@ def _number(d, k, cur):
     return _fv(d[k])
 
 
-def material_from(d):
+def material_from(d, images=None):
     if not isinstance(d, dict):
         raise SceneError("Material should be contained in Dson::Dict")
     if "type" not in d:
@@ -532,16 +543,23 @@ def material_from(d):
         fres, metal, spec = s["fresnel"], s["metal"], s["spec"]
         div = f32(1) if metal == f32(1) else f32(1) - metal
         diffuse = vmul(s["diffuse"], f32(1) / div)
-        fres = _colored(d, "fresnel", "fresnel_texture", "fresnel_tiling", "fresnel_offset", fres)
-        diffuse = _colored(d, "diffuse", "diffuse_texture", "diffuse_tiling", "diffuse_offset", diffuse)
-        _colored(d, "normal_color", "normal_texture", "normal_tiling", "normal_offset", None)
-        _colored(d, "alpha_cutout_color", "alpha_cutout_texture", "alpha_cutout_tiling", "alpha_cutout_offset", None)
-        _number(d, "alpha_cutout_threshold", f32(0.5))
+        fres, m_color = _colored(d, "fresnel", "fresnel_texture", "fresnel_tiling", "fresnel_offset", fres, images)
+        diffuse, m_diff = _colored(d, "diffuse", "diffuse_texture", "diffuse_tiling", "diffuse_offset", diffuse,
+                                   images)
+        ncol, m_norm = _colored(d, "normal_color", "normal_texture", "normal_tiling", "normal_offset", V(1, 1, 1),
+                                images)
+        _, m_alpha = _colored(d, "alpha_cutout_color", "alpha_cutout_texture", "alpha_cutout_tiling",
+                              "alpha_cutout_offset", V(1, 1, 1), images)
+        thresh = _number(d, "alpha_cutout_threshold", f32(0.5))
         spec = _number(d, "specular", spec)
         metal = _number(d, "metal", metal)
         s = _dumb_based_ctor(fres, spec, diffuse, metal)
-        return {"kind": "dumb_based", "albedo": tuple(float(c) for c in s["fresnel"]),
-                "diffuse": tuple(float(c) for c in s["diffuse"]), "p": (float(s["spec"]), float(s["metal"]), 0.0)}
+        out = {"kind": "dumb_based", "albedo": tuple(float(c) for c in s["fresnel"]),
+               "diffuse": tuple(float(c) for c in s["diffuse"]), "p": (float(s["spec"]), float(s["metal"]), 0.0)}
+        if any(m is not None for m in (m_color, m_diff, m_norm, m_alpha)):
+            out["maps"] = {"color": m_color, "diffuse": m_diff, "normal": m_norm, "alpha": m_alpha,
+                           "normal_color": tuple(float(c) for c in ncol), "alpha_threshold": float(thresh)}
+        return out
     if t == "simple_stochastic":
         def ctor(color, diffuse, smooth, shine):
             total = f32(diffuse) + f32(smooth)
@@ -562,8 +580,12 @@ def material_from(d):
                    "aluminum": (0.91, 0.92, 0.92), "silver": (0.95, 0.93, 0.88)}
         if isinstance(d.get("engine_color"), Str) and d["engine_color"] in engines:
             color = V(*engines[d["engine_color"]])
-        color = _colored(d, "color", "texture", "tiling", "offset", color)
-        return {"kind": "fresnel", "albedo": tuple(float(c) for c in color)}
+        color, m_color = _colored(d, "color", "texture", "tiling", "offset", color, images)
+        out = {"kind": "fresnel", "albedo": tuple(float(c) for c in color)}
+        if m_color is not None:
+            out["maps"] = {"color": m_color, "diffuse": None, "normal": None, "alpha": None,
+                           "normal_color": (1.0, 1.0, 1.0), "alpha_threshold": 0.5}
+        return out
     raise SceneError('Unknown material type: "' + t + '"')
 
 
@@ -676,6 +698,7 @@ class SceneFile:
         self.lens_x = None
         self.max_bounces, self.fsaa, self.boxing, self.ignore_backfaces = 2, (1, 1), 0, True
         self.lens, self.lens_params = 0, [f32(0)] * 8
+        self.textures, self.texture_names = [], {}
 
     # -------------------------------------------------------------- files
     def load(self, path, depth=0):
@@ -744,8 +767,48 @@ class SceneFile:
         self.obj_files[key] = dict(zip(names, meshes))
 
     # -------------------------------------------------------------- sections
+    def image_id(self, v):
+        """DumbRenderContext::getImageId: -1, a texture named before, or a new texture from a PNG
+        file (decoded with Pillow to RGBA, byte / 255)."""
+        if isinstance(v, (bool, float)) or v is None:
+            return -1
+        if isinstance(v, Str):
+            for key in ("name::" + v, "png::" + v):
+                if key in self.texture_names:
+                    return self.texture_names[key]
+            raise SceneError('Texture "' + v + '" not found')
+        if not isinstance(v, dict):
+            raise SceneError("Unsupported dson type for texture")
+        if "png" not in v:
+            raise SceneError("Image dict incomplete")
+        if not isinstance(v["png"], Str):
+            raise SceneError("Image 'png' entry MUST BE a string")
+        from PIL import Image
+
+        name = v["png"]
+        path = self.source_path + name
+        if not os.path.isfile(path):
+            path = name
+        try:
+            rgba = np.asarray(Image.open(path).convert("RGBA"))
+        except Exception:
+            raise SceneError('Could not read file: "' + path + '"') from None
+        tid = len(self.textures)
+        self.textures.append({"rgba": rgba.astype(np.float32) / f32(255), "filter": 1})
+        self.texture_names["png::" + path] = tid
+        self.texture_names["png::" + name] = tid
+        if "filtering" in v:
+            if not isinstance(v["filtering"], Str) or v["filtering"] not in ("none", "bilinear"):
+                raise SceneError("Image filter can be only none/[bilinear]")
+            self.textures[tid]["filter"] = 0 if v["filtering"] == "none" else 1
+        if "name" in v:
+            if not isinstance(v["name"], Str):
+                raise SceneError("Image name entry MUST BE a string")
+            self.texture_names["name::" + v["name"]] = tid
+        return tid
+
     def add_material(self, d):
-        m = material_from(d)
+        m = material_from(d, self)
         self.materials.append(m)
         mid = len(self.materials) - 1
         if isinstance(d.get("name"), Str):
@@ -849,7 +912,8 @@ class SceneFile:
         out, t = [], item["t"]
         for mesh in chosen:
             for tri in bake(mesh):
-                out.append([[t.point(c[0]) for c in tri], [t.normal(c[1]) for c in tri], item["mat"]])
+                out.append([[t.point(c[0]) for c in tri], [t.normal(c[1]) for c in tri], item["mat"],
+                            [c[2] for c in tri]])
         return out
 
     @staticmethod
@@ -986,6 +1050,10 @@ class SceneFile:
     @property
     def mat_id(self):
         return np.array([u[2] for u in self.units], np.int32)
+
+    @property
+    def uv(self):
+        return np.array([[c for t in u[3] for c in t[:2]] for u in self.units], np.float32).reshape(-1, 6)
 
     @property
     def cam_rows(self):

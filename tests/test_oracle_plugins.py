@@ -441,3 +441,139 @@ def test_spherical_segment_lens_filter_scales_the_image(oracle):
     b = mk((0.5, 0.25, 2.0)).render(nthreads=4)
     assert a.max() > 0
     np.testing.assert_allclose(b, a * np.array([0.5, 0.25, 2.0], np.float32), rtol=1e-6)
+
+
+# ---------------------------------------------------------------- textures
+def _tex(rgb_rows, filtering="none"):
+    a = np.array(rgb_rows, np.float32)
+    if a.shape[-1] == 3:
+        a = np.concatenate([a, np.ones(a.shape[:2] + (1,), np.float32)], -1)
+    return PL.texture(a, filtering)
+
+
+def test_texture_sample_closed_forms(oracle):
+    red, blue, green, white = (1, 0, 0), (0, 0, 1), (0, 1, 0), (1, 1, 1)
+    row = _tex([[red, blue]], "none")
+    d, t = oracle.pack_textures([row])
+    s = lambda x, y: oracle.texture_sample(d[0], t, x, y)[:3].tolist()
+    # nearest: the texel of the 2x2 cell the offset is closer to
+    assert s(0.1, 0.5) == [1, 0, 0] and s(0.4, 0.5) == [0, 0, 1] and s(0.6, 0.5) == [0, 0, 1]
+    assert s(0.9, 0.5) == [1, 0, 0]                         # wraps to texel 0
+    assert s(1.1, 0.5) == s(0.1, 0.5) and s(-0.6, 0.5) == s(0.4, 0.5)
+    bil = _tex([[red, blue]], "bilinear")
+    d, t = oracle.pack_textures([bil])
+    np.testing.assert_allclose(oracle.texture_sample(d[0], t, 0.25, 0.5)[:3], [0.5, 0, 0.5], atol=1e-7)
+    np.testing.assert_allclose(oracle.texture_sample(d[0], t, 0.1, 0.5)[:3], [0.8, 0, 0.2], atol=1e-6)
+    # y runs up: row 0 (the top of the image) is sampled near v = 1
+    col = _tex([[red], [blue]], "none")
+    d, t = oracle.pack_textures([col])
+    assert oracle.texture_sample(d[0], t, 0.5, 0.9)[:3].tolist() == [1, 0, 0]
+    assert oracle.texture_sample(d[0], t, 0.5, 0.4)[:3].tolist() == [0, 0, 1]
+    # bilinear interior points against a float64 statement of the same grid
+    rng = np.random.default_rng(1)
+    img = rng.uniform(0, 1, (5, 7, 4)).astype(np.float32)
+    d, t = oracle.pack_textures([PL.texture(img, "bilinear")])
+    for _ in range(200):
+        x, y = rng.uniform(-3, 3, 2)
+        fx, fy = x - math.floor(x), 1.0 - (y - math.floor(y))
+        px, py = fx * 7, fy * 5
+        x0, y0 = int(px) % 7, int(py) % 5
+        ox, oy = px - int(px), py - int(py)
+        x1, y1 = (x0 + 1) % 7, (y0 + 1) % 5
+        want = (img[y0, x0] * (1 - ox) * (1 - oy) + img[y1, x0] * (1 - ox) * oy + img[y0, x1] * ox * (1 - oy)
+                + img[y1, x1] * ox * oy)
+        if min(ox, 1 - ox, oy, 1 - oy) < 1e-4:
+            continue                                        # float rounding of the cell edge
+        np.testing.assert_allclose(oracle.texture_sample(d[0], t, x, y), want, atol=2e-5)
+
+
+def _tex_floor_render(oracle, material, textures=None, uv=None, W=16, H=12, extra_tris=None, extra_mats=None,
+                      bounces=0):
+    tris = _floor() if extra_tris is None else np.concatenate([_floor(), extra_tris])
+    mats = [material] + (extra_mats or [])
+    mid = [0, 0] + ([1] * (len(tris) - 2))
+    t = oracle.Octree(tris, 16, 12)
+    R = oracle.Renderer(t, mid, mats, [PL.directional_light((1.5, 1.0, 0.5), (0, -1, 0), 50.0)],
+                        _camera_down(3.0).as_array(), W, H, (1, 1), bounces, lens=1, ignore_backfaces=False,
+                        textures=textures, uv=uv)
+    return R.render_ref(nthreads=4)
+
+
+def _floor_uv(n=2):
+    # floor vertices (-20,0,-20) (20,0,-20) (-20,0,20) / (-20,0,20) (20,0,-20) (20,0,20): u = (x+20)/40, v = (z+20)/40
+    tris = _floor().reshape(-1, 3, 3)
+    uv = np.stack([(tris[..., 0] + 20) / 40, (tris[..., 2] + 20) / 40], -1).reshape(-1, 6)
+    return uv.astype(np.float32)
+
+
+def test_constant_diffuse_texture_scales_the_diffuse_colour(oracle):
+    base = PL.dumb_based_shader((0.0, 0.0, 0.0), 8.0, (0.8, 0.6, 0.4), 0.0)
+    c = (0.5, 0.25, 0.75)
+    tex = [_tex([[c]], "bilinear")]
+    textured = PL.with_maps(base, diffuse=PL.texture_map(0, (3.0, 2.0), (0.1, 0.2)))
+    a = _tex_floor_render(oracle, textured, tex, _floor_uv())
+    scaled = PL.dumb_based_shader((0.0, 0.0, 0.0), 8.0,
+                                  tuple(float(np.float32(x) * np.float32(y)) for x, y in zip((0.8, 0.6, 0.4), c)), 0.0)
+    b = _tex_floor_render(oracle, scaled)
+    assert a.max() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-7)
+
+
+def test_checker_texture_follows_the_texture_coordinates(oracle):
+    """A 2x2 checker on the floor with u, v from x, z: each pixel shows the checker colour of the
+    floor point its camera ray meets."""
+    A, B = (1.0, 0.2, 0.2), (0.2, 0.2, 1.0)
+    tex = [_tex([[A, B], [B, A]], "none")]
+    m = PL.with_maps(PL.dumb_based_shader((0.0, 0.0, 0.0), 8.0, (1.0, 1.0, 1.0), 0.0),
+                     diffuse=PL.texture_map(0, (8.0, 8.0)))
+    W, H = 24, 18
+    img = _tex_floor_render(oracle, m, tex, _floor_uv(), W, H)
+    dirs = _camera_dirs(_camera_down(3.0), W, H)
+    hit = np.array([0.0, 3.0, 0.0]) + dirs * (3.0 / -dirs[..., 1:2])       # floor points
+    u, v = (hit[..., 0] + 20) / 40 * 8, (hit[..., 2] + 20) / 40 * 8
+    # nearest texel of the 2x2 texture: column from u, row from 1 - v (row 0 at the top)
+    col = np.floor(((u % 1.0) * 2) + 0.5).astype(int) % 2
+    row = np.floor(((1.0 - (v % 1.0)) * 2) + 0.5).astype(int) % 2
+    expect_A = (col == row)
+    cos = 1.0
+    want = np.where(expect_A[..., None], np.array(A), np.array(B)) * cos * np.array([1.5, 1.0, 0.5])
+    frac_u, frac_v = (u * 2) % 1.0, (v * 2) % 1.0
+    clear = (np.abs(frac_u - 0.5) > 0.02) & (np.abs(frac_v - 0.5) > 0.02)   # away from texel edges
+    assert clear.mean() > 0.5
+    np.testing.assert_allclose(img[clear], np.clip(want[clear], 0, None), rtol=1e-5, atol=1e-6)
+
+
+def test_alpha_cutout_sheet_lets_light_and_view_through(oracle):
+    """A sheet cut out everywhere: camera rays continue through it as its indirect sample
+    (type 3, colour passed on unchanged) and light photons pass it, so the floor looks as if
+    the sheet were not there (one bounce: the floor seen through the sheet is one layer
+    deeper and keeps its direct light)."""
+    floor = PL.dumb_based_shader((0.0, 0.0, 0.0), 8.0, (0.8, 0.6, 0.4), 0.0)
+    sheet = PL.dumb_based_shader((0.2, 0.2, 0.2), 8.0, (0.9, 0.9, 0.9), 0.0)
+    cut = [_tex(np.zeros((1, 1, 4), np.float32), "none")]                  # alpha 0 everywhere
+    sheet_tris = _floor(2.0, half=0.8)
+    uv = np.zeros((4, 6), np.float32)
+    alone = _tex_floor_render(oracle, floor, bounces=1)
+    with_cut = _tex_floor_render(oracle, floor, cut, uv, extra_tris=sheet_tris, bounces=1,
+                                 extra_mats=[PL.with_maps(sheet, alpha=PL.texture_map(0), alpha_threshold=0.5)])
+    assert alone.max() > 0
+    np.testing.assert_allclose(with_cut, alone, rtol=1e-5, atol=1e-7)
+    solid = [_tex(np.ones((1, 1, 4), np.float32), "none")]                 # alpha 1: above the threshold
+    with_solid = _tex_floor_render(oracle, floor, solid, uv, extra_tris=sheet_tris, bounces=1,
+                                   extra_mats=[PL.with_maps(sheet, alpha=PL.texture_map(0), alpha_threshold=0.5)])
+    assert np.abs(with_solid - alone).max() > 1e-3
+    # without a bounce the cut-out sheet shows black: passing through is an indirect sample
+    black = _tex_floor_render(oracle, floor, cut, uv, extra_tris=sheet_tris, bounces=0,
+                              extra_mats=[PL.with_maps(sheet, alpha=PL.texture_map(0), alpha_threshold=0.5)])
+    assert black.min() == 0.0 and (black.sum(-1) == 0).mean() > 0.2
+
+
+def test_flat_normal_map_keeps_the_normal_and_a_tilted_one_turns_it(oracle):
+    base = PL.dumb_based_shader((0.3, 0.3, 0.3), 16.0, (0.6, 0.6, 0.6), 0.4)
+    plain = _tex_floor_render(oracle, base)
+    flat = [_tex([[(0.5, 0.5, 1.0)]], "bilinear")]          # (r, g, b) = (0.5, 0.5, 1): weights (1, 0, 0)
+    a = _tex_floor_render(oracle, PL.with_maps(base, normal=PL.texture_map(0)), flat, _floor_uv())
+    np.testing.assert_allclose(a, plain, rtol=1e-6, atol=1e-7)
+    tilted = [_tex([[(0.9, 0.5, 0.6)]], "bilinear")]
+    b = _tex_floor_render(oracle, PL.with_maps(base, normal=PL.texture_map(0)), tilted, _floor_uv())
+    assert np.abs(b - plain).max() > 1e-3

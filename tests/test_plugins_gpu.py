@@ -168,3 +168,45 @@ def test_spherical_segment_lens_matches_oracle(dev, oracle, plugins):
     ref = o.render(nthreads=8) if plugins == "benchmark" else o.render_ref(nthreads=8)
     assert ref.max() > 0
     _close(img, ref)
+
+
+
+def _textures():
+    rng = np.random.default_rng(9)
+    checker = np.kron(np.array([[1.0, 0.2], [0.2, 1.0]]), np.ones((4, 4)))[..., None] * np.array([1.0, 0.8, 0.5, 1.0])
+    noise = np.concatenate([rng.uniform(0.35, 0.65, (16, 16, 2)), rng.uniform(0.8, 1.0, (16, 16, 1)),
+                            np.ones((16, 16, 1))], -1)
+    mask = np.ones((8, 8, 4))
+    mask[..., 3] = (np.add.outer(np.arange(8), np.arange(8)) % 3 != 0)
+    return [PL.texture(checker.astype(np.float32), "none"), PL.texture(noise.astype(np.float32), "bilinear"),
+            PL.texture(mask.astype(np.float32), "none")]
+
+
+@pytest.mark.parametrize("which", ["diffuse+normal", "fresnel+alpha", "fresnel_shader"])
+def test_textured_materials_match_oracle(dev, oracle, which):
+    sc = S.spheres_scene(6, seed=8)
+    mats = []
+    for i, m in enumerate(sc.materials):
+        base = PL.dumb_based_shader((0.6, 0.5, 0.4), 16.0, m["albedo"], 0.3)
+        if which == "diffuse+normal":
+            mats.append(PL.with_maps(base, diffuse=PL.texture_map(0, (3.0, 2.0), (0.1, 0.0)),
+                                     normal=PL.texture_map(1), normal_color=(1.0, 0.7, 0.9)))
+        elif which == "fresnel+alpha":
+            mats.append(PL.with_maps(base, color=PL.texture_map(1, (2.0, 2.0)), alpha=PL.texture_map(2, (5.0, 5.0)),
+                                     alpha_threshold=0.5) if i % 2 else base)
+        else:
+            mats.append(PL.with_maps(PL.fresnel_shader(m["albedo"]), color=PL.texture_map(0, (4.0, 4.0)))
+                        if i % 2 else base)
+    sc.materials = mats
+    sc.lights = LIGHTS[:3]
+    # texture coordinates from the vertex positions
+    v = sc.tris.reshape(-1, 3, 3)
+    uv = np.stack([v[..., 0] * 0.37 + v[..., 2] * 0.11, v[..., 1] * 0.29 - v[..., 2] * 0.21], -1).reshape(-1, 6)
+    uv = uv.astype(np.float32)
+    tex = _textures()
+    print(f"[case] textured materials {which} 40x30 spp 2x2 depth 3", flush=True)
+    g, o = _pair(oracle, sc, 40, 30, (2, 2), 3, textures=tex, uv=uv, ignore_backfaces=False)
+    img = _gpu(g)
+    ref = o.render_ref(nthreads=8)
+    assert ref.max() > 0
+    _close(img, ref)

@@ -39,7 +39,8 @@ def _pair(path, W, H, iterations=1, **over):
     o = oracle.Renderer(oracle.Octree(sf.tris, 8, 24), sf.mat_id, sf.materials, sf.lights, sf.cam, W, H,
                         over.get("samples_per_pixel", sf.fsaa), over.get("max_bounces", sf.max_bounces),
                         ignore_backfaces=sf.ignore_backfaces, normals=sf.normals, boxing=boxing, lens=sf.lens,
-                        lens_params=sf.lens_params, cam_rows=sf.cam_rows)
+                        lens_params=sf.lens_params, cam_rows=sf.cam_rows,
+                        textures=sf.textures if sf.textures else None, uv=sf.uv)
     frame = None
     for it in range(1, iterations + 1):
         print(f"[case] {os.path.basename(path)} {W}x{H} iteration {it} {over}", flush=True)
@@ -132,3 +133,13 @@ def test_example_scene_spherical_segment_lens_matches_oracle(dev, tmp_path):
     path.write_text(text)
     g, _ = _pair(str(path), 64, 40)
     assert g.lens == D.plugins.SPHERICAL_SEGMENT_LENS
+
+
+def test_textured_scene_matches_oracle(dev, tmp_path):
+    """PNG textures through the scene file: a nearest-filtered checker diffuse map with a normal
+    map, an alpha-cut leaf with a fresnel-colour texture, a textured FresnelShader."""
+    from test_scene_file import write_textured_scene
+
+    path = write_textured_scene(tmp_path)
+    g, _ = _pair(path, 48, 32, iterations=2)
+    assert g.ntextures == 3 and g.nmaps == 3
