@@ -1,17 +1,17 @@
 // Host octree builder: Octree<Renderable<BakedTriFace>>::push(all) + build()
-// (Octree.impl.cuh:97-129, 339-533) re-designed as a parallel work-queue
-// builder whose output is the device layout the traversal kernel reads.
+// (Octree.impl.cuh:97-129, 339-533) re-designed as a parallel builder (large nodes
+// split by the whole thread team, the subtrees below them built one per thread)
+// whose output is the device layout the traversal kernel reads.
 //
 // The tree it produces (node boxes, leaf lists in stored order, child order)
 // is the reference's tree; only node *indices* differ, exactly as in the
 // reference's own thread-pool build (scheduling permutes indices, the data
 // fixes everything else).  Splits are independent once a node's triangle list
-// is final, so every queued node is split by whichever worker takes it.
+// is final, so every subtree is built by whichever thread takes it.
 #include "dr_octree_build.h"
 
 #include <algorithm>
 #include <atomic>
-#include <condition_variable>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -189,35 +189,59 @@ struct BNode {
 class NodePool {
  public:
   static constexpr int kShift = 13, kChunk = 1 << kShift, kMaxChunks = 1 << 17;
+  // A run of nodes one thread allocates from without locking, refilled a chunk at a time.
+  struct Cursor {
+    int32_t next = 0, end = 0;
+  };
   NodePool() : chunks_(new std::unique_ptr<BNode[]>[kMaxChunks]) {}
   BNode &operator[](int32_t i) { return chunks_[i >> kShift][i & (kChunk - 1)]; }
-  int32_t alloc(int k) {           // k consecutive nodes (1 or 8), never straddling a chunk
-    std::lock_guard<std::mutex> g(mu_);
-    if ((size_ & (kChunk - 1)) + k > kChunk) size_ = (size_ + kChunk - 1) & ~(kChunk - 1);
-    const int32_t first = size_;
-    const int c = first >> kShift;
-    if (c >= kMaxChunks) return -1;
-    if (!chunks_[c]) chunks_[c].reset(new BNode[kChunk]);
-    size_ += k;
+  int32_t alloc(int k, Cursor &cur) {   // k consecutive nodes (1 or 8), never straddling a chunk
+    if (cur.next + k > cur.end) {
+      std::lock_guard<std::mutex> g(mu_);
+      const int c = used_;
+      if (c >= kMaxChunks) return -1;
+      chunks_[c].reset(new BNode[kChunk]);
+      used_ = c + 1;
+      cur.next = c << kShift;
+      cur.end = cur.next + kChunk;
+    }
+    const int32_t first = cur.next;
+    cur.next += k;
     return first;
   }
-  int32_t size() const { return size_; }
+  // upper bound of the node indices handed out (unused slots are never reached from the root)
+  int32_t size() const { return used_ << kShift; }
  private:
   std::unique_ptr<std::unique_ptr<BNode[]>[]> chunks_;
   std::mutex mu_;
-  int32_t size_ = 0;
+  int used_ = 0;
 };
+
+// Runs fn(begin, end) over [0, n) in contiguous chunks of at least `grain` elements on
+// up to `threads` threads (the caller's thread takes the first chunk).  Used inside the split of a large node, where the
+// per-triangle clipping dominates; every result is written per element and combined
+// afterwards in element order, so the tree does not depend on the chunking.
+template <class Fn>
+void parallel_chunks(int n, int threads, Fn fn, int grain = 4096) {
+  threads = std::max(1, std::min(threads, n / grain + 1));
+  if (threads == 1) { fn(0, n); return; }
+  std::vector<std::thread> team;
+  const int step = (n + threads - 1) / threads;
+  for (int t = 1; t < threads; t++) {
+    const int b = t * step, e = std::min(n, b + step);
+    if (b < e) team.emplace_back([=] { fn(b, e); });
+  }
+  fn(0, std::min(n, step));
+  for (auto &th : team) th.join();
+}
 
 struct Builder {
   const Tri *tris;
   int thresh, maxdepth;
   V3 Ev;
+  int threads = 1;                    // team size for the split of a large node
+  static constexpr int kBigNode = 16384;
   NodePool pool;
-  // work queue (LIFO keeps workers depth-first, like the reference's recursion)
-  std::mutex qmu;
-  std::condition_variable qcv;
-  std::vector<std::pair<int32_t, int>> q;
-  int active = 0;
   std::atomic<bool> failed{false};
 
   // splitAABB (Octree.impl.cuh:423-434)
@@ -233,13 +257,30 @@ struct Builder {
     r[6] = box(v3(c.x - E, c.y - E, s.z), v3(e.x, e.y, c.z + E));
     r[7] = box(c - Ev, e);
   }
-  // reduceNode(index, true) on a leaf (Octree.impl.cuh:489-533, leaf branch)
+  // reduceNode(index, true) on a leaf (Octree.impl.cuh:489-533, leaf branch); min / max
+  // are exact, so a large leaf's bounds may be gathered in chunks and merged.
   void reduce_leaf(BNode &nd, ClipBuf &l, ClipBuf &tmp) const {
+    const int n = (int)nd.list.size();
+    auto gather = [&](int b, int e, ClipBuf &cl, ClipBuf &ct, V3 &start, V3 &end) {
+      for (int j = b; j < e; j++) {
+        const Box ob = isect_bounds(nd.box, tris[nd.list[j]], cl, ct);
+        start = v3(smin(start.x, ob.s.x), smin(start.y, ob.s.y), smin(start.z, ob.s.z));
+        end = v3(smax(end.x, ob.e.x), smax(end.y, ob.e.y), smax(end.z, ob.e.z));
+      }
+    };
     V3 start = nd.box.e, end = nd.box.s;
-    for (int32_t id : nd.list) {
-      const Box ob = isect_bounds(nd.box, tris[id], l, tmp);
-      start = v3(smin(start.x, ob.s.x), smin(start.y, ob.s.y), smin(start.z, ob.s.z));
-      end = v3(smax(end.x, ob.e.x), smax(end.y, ob.e.y), smax(end.z, ob.e.z));
+    if (n < kBigNode || threads <= 1) {
+      gather(0, n, l, tmp, start, end);
+    } else {
+      std::mutex mu;
+      parallel_chunks(n, threads, [&](int b, int e) {
+        ClipBuf cl, ct;
+        V3 s0 = nd.box.e, e0 = nd.box.s;
+        gather(b, e, cl, ct, s0, e0);
+        std::lock_guard<std::mutex> g(mu);
+        start = v3(smin(start.x, s0.x), smin(start.y, s0.y), smin(start.z, s0.z));
+        end = v3(smax(end.x, e0.x), smax(end.y, e0.y), smax(end.z, e0.z));
+      });
     }
     finish_reduce(nd, start, end);
   }
@@ -251,65 +292,118 @@ struct Builder {
     end = v3(smin(end.x, be.x), smin(end.y, be.y), smin(end.z, be.z));
     nd.box = box(start, end);
   }
-  // split (Octree.impl.cuh:339-421) of one node; enqueues its children.
+  // split (Octree.impl.cuh:339-421) of one node; appends its children to `out`.
   void split(int32_t idx, int depth, ClipBuf &l, ClipBuf &tmp, std::vector<uint8_t> &cache,
-             std::vector<std::pair<int32_t, int>> &out) {
+             std::vector<std::pair<int32_t, int>> &out, NodePool::Cursor &cur) {
     BNode &nd = pool[idx];
     const int n = (int)nd.list.size();
     if (depth >= maxdepth || n < thresh) { reduce_leaf(nd, l, tmp); return; }
     V3 center = v3(0, 0, 0);
     const Box bounds = nd.box;
-    for (int i = 0; i < n; i++) center = center + isect_center(bounds, tris[nd.list[i]], l, tmp);
-    center = divf(center, (float)n);
     Box sub[8];
-    split_aabb(bounds, center, sub);
-    // splittingMakesSence with the intersection cache (Octree.impl.cuh:437-475)
+    int insiders[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     cache.assign(n, 0);
+    if (n < kBigNode || threads <= 1) {
+      for (int i = 0; i < n; i++) center = center + isect_center(bounds, tris[nd.list[i]], l, tmp);
+      center = divf(center, (float)n);
+      split_aabb(bounds, center, sub);
+      for (int i = 0; i < 8; i++)
+        for (int j = 0; j < n; j++)
+          if (box_hits_tri(sub[i], tris[nd.list[j]], 0)) { insiders[i]++; cache[j] |= (uint8_t)(1u << i); }
+    } else {
+      // the clipped centres per element in parallel, summed in element order as the loop above
+      std::vector<V3> part(n);
+      parallel_chunks(n, threads, [&](int b, int e) {
+        ClipBuf cl, ct;
+        for (int j = b; j < e; j++) part[j] = isect_center(bounds, tris[nd.list[j]], cl, ct);
+      });
+      for (int i = 0; i < n; i++) center = center + part[i];
+      center = divf(center, (float)n);
+      split_aabb(bounds, center, sub);
+      std::mutex mu;
+      parallel_chunks(n, threads, [&](int b, int e) {
+        int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = b; j < e; j++)
+          for (int i = 0; i < 8; i++)
+            if (box_hits_tri(sub[i], tris[nd.list[j]], 0)) { cnt[i]++; cache[j] |= (uint8_t)(1u << i); }
+        std::lock_guard<std::mutex> g(mu);
+        for (int i = 0; i < 8; i++) insiders[i] += cnt[i];
+      });
+    }
+    // splittingMakesSence with the intersection cache (Octree.impl.cuh:437-475)
     int full = 0;
     float load = 0.0f;
     for (int i = 0; i < 8; i++) {
-      int insiders = 0;
-      for (int j = 0; j < n; j++)
-        if (box_hits_tri(sub[i], tris[nd.list[j]], 0)) { insiders++; cache[j] |= (uint8_t)(1u << i); }
-      if (insiders >= n) full++;
-      load += ((float)insiders) / ((float)n);
+      if (insiders[i] >= n) full++;
+      load += ((float)insiders[i]) / ((float)n);
     }
     if (full >= 1 || !(load < smin(((float)n) / ((float)thresh), 6.0f))) { reduce_leaf(nd, l, tmp); return; }
     // splitNode + distribution (Octree.impl.cuh:388-404, 477-486)
-    const int32_t c0 = pool.alloc(8);
+    const int32_t c0 = pool.alloc(8, cur);
     if (c0 < 0) { failed = true; return; }
     for (int i = 0; i < 8; i++) pool[c0 + i].box = sub[i];
-    for (int j = 0; j < n; j++) {
-      const int32_t id = nd.list[j];
-      for (int i = 0; i < 8; i++) if (cache[j] & (1u << i)) pool[c0 + i].list.push_back(id);
+    if (n < kBigNode || threads <= 1) {
+      for (int j = 0; j < n; j++) {
+        const int32_t id = nd.list[j];
+        for (int i = 0; i < 8; i++) if (cache[j] & (1u << i)) pool[c0 + i].list.push_back(id);
+      }
+    } else {
+      // one child per task, each list still in the parent's order
+      parallel_chunks(8, threads, [&](int b, int e) {
+        for (int i = b; i < e; i++) {
+          std::vector<int32_t> &cl = pool[c0 + i].list;
+          cl.reserve(insiders[i]);
+          for (int j = 0; j < n; j++) if (cache[j] & (1u << i)) cl.push_back(nd.list[j]);
+        }
+      }, 1);
     }
     std::vector<int32_t>().swap(nd.list);
     nd.child = c0;
     for (int i = 7; i >= 0; i--) out.emplace_back(c0 + i, depth + 1);
   }
-  void worker() {
+  // The subtree below one node, depth first on the calling thread (the order of the
+  // reference's recursion; node indices do not affect the tree).
+  void build_subtree(std::pair<int32_t, int> task, NodePool::Cursor &cur) {
     ClipBuf l, tmp;
     std::vector<uint8_t> cache;
-    std::vector<std::pair<int32_t, int>> out;
-    for (;;) {
-      std::pair<int32_t, int> task;
-      {
-        std::unique_lock<std::mutex> g(qmu);
-        qcv.wait(g, [&] { return !q.empty() || active == 0; });
-        if (q.empty()) { qcv.notify_all(); return; }
-        task = q.back();
-        q.pop_back();
-        active++;
-      }
+    std::vector<std::pair<int32_t, int>> stack{task}, out;
+    while (!stack.empty() && !failed) {
+      const auto t = stack.back();
+      stack.pop_back();
       out.clear();
-      split(task.first, task.second, l, tmp, cache, out);
-      {
-        std::lock_guard<std::mutex> g(qmu);
-        q.insert(q.end(), out.begin(), out.end());
-        active--;
-      }
-      qcv.notify_all();
+      split(t.first, t.second, l, tmp, cache, out, cur);
+      stack.insert(stack.end(), out.begin(), out.end());
     }
+  }
+  // Large nodes first, one at a time with the whole team inside each split; then the
+  // subtrees below them, handed out to the threads largest first.
+  void run(int32_t root) {
+    NodePool::Cursor cur;
+    std::vector<std::pair<int32_t, int>> frontier{{root, 0}}, small, out;
+    ClipBuf l, tmp;
+    std::vector<uint8_t> cache;
+    while (!frontier.empty() && !failed) {
+      std::vector<std::pair<int32_t, int>> next;
+      for (const auto &t : frontier) {
+        if ((int)pool[t.first].list.size() < kBigNode) { small.push_back(t); continue; }
+        out.clear();
+        split(t.first, t.second, l, tmp, cache, out, cur);
+        next.insert(next.end(), out.begin(), out.end());
+      }
+      frontier.swap(next);
+    }
+    std::stable_sort(small.begin(), small.end(), [&](const std::pair<int32_t, int> &a, const std::pair<int32_t, int> &b) {
+      return pool[a.first].list.size() > pool[b.first].list.size();
+    });
+    std::atomic<size_t> at{0};
+    auto work = [&] {
+      NodePool::Cursor c;
+      for (size_t i = at++; i < small.size() && !failed; i = at++) build_subtree(small[i], c);
+    };
+    std::vector<std::thread> team;
+    for (int i = 1; i < threads; i++) team.emplace_back(work);
+    work();
+    for (auto &th : team) th.join();
   }
   // reduceNode(0, false) (Octree.impl.cuh:489-533): interior boxes from their children
   void reduce_interior(int32_t idx) {
@@ -343,7 +437,11 @@ int build_octree(const float *tris9, int ntris, int split_threshold, int max_dep
   B.thresh = split_threshold;
   B.maxdepth = max_depth;
   B.Ev = v3(kVecEps, kVecEps, kVecEps) * 256;        // EPSILON_VECTOR * 256
-  const int32_t root = B.pool.alloc(1);
+  if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+  nthreads = std::min(nthreads, 64);
+  B.threads = nthreads;
+  NodePool::Cursor rc0;
+  const int32_t root = B.pool.alloc(1, rc0);
   BNode &r = B.pool[root];
   r.box = box(v3(0, 0, 0), v3(0, 0, 0));
   // push (Octree.impl.cuh:97-111): first element sets the root box, later ones union box +- eps
@@ -359,14 +457,7 @@ int build_octree(const float *tris9, int ntris, int split_threshold, int max_dep
     }
     r.list[i] = i;
   }
-  if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
-  nthreads = std::min(nthreads, 64);
-  B.q.emplace_back(root, 0);
-  {
-    std::vector<std::thread> th;
-    for (int i = 0; i < nthreads; i++) th.emplace_back([&B] { B.worker(); });
-    for (auto &t : th) t.join();
-  }
+  B.run(root);
   if (B.failed) return -2;
   B.reduce_interior(root);
 

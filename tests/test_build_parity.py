@@ -25,6 +25,8 @@ CASES = [
     ("soup", lambda: S.random_soup(3000, seed=1, extent=10.0, size=1.5), 16, 12),
     ("soup_t8_d24", lambda: S.random_soup(2000, seed=2, extent=5.0, size=2.0), 8, 24),
     ("heightfield128", lambda: S.heightfield(128, seed=1), 16, 12),
+    # several nodes above the builder's team-split size (16384 triangles) at depths 0-2
+    ("soup80k_t8_d24", lambda: S.random_soup(80000, seed=3, extent=20.0, size=1.0), 8, 24),
     ("spheres", lambda: S.spheres_scene(8, seed=0).tris, 16, 12),
     ("icosphere4", lambda: S.icosphere((0.5, -0.2, 0.1), 1.3, 4, displace=0.05, seed=1), 16, 12),
     ("single", lambda: S.random_soup(1, seed=1), 16, 12),
