@@ -165,17 +165,18 @@ struct BenchPath {
 // direction) as its colour when that colour is not black (:396-413).  Colours are
 // carried forward: L += beta * (getReflectedColor of each direct photon), and the
 // child layer's throughput is beta * getReflectedColor(indirect photon) * sampleWeight.
+//
+// Only what the shadow casts need stays live across them: the vertex (triangle, point,
+// incoming direction, material) and the photon.  The shading record and the indirect
+// sample are computed again where they are used; both are functions of the vertex and
+// of counter-based random numbers, so the order of evaluation does not change them.
 struct RefPath {
   uint32_t pix, samp;
   int layer, phase, light, lk, excl, trans;   // lk: photons of the current light still to send
   V3 o, d;                          // the next ray to cast
   float beta[3], L[3];
-  int h, mid, stype;                // current vertex
-  V3 din, nu;
-  RefHit H;
-  bool has;
-  V3 sdir;
-  float bnext[3];                   // throughput of the child layer
+  int h, mid;                       // current vertex: triangle, material
+  V3 hp, din;                       //   hit point, incoming ray direction
   V3 pd;                            // current photon: direction, colour, unit direction?
   float pc[3];
   bool unit;
@@ -200,76 +201,94 @@ struct RefPath {
   __device__ __forceinline__ int ray_excl() const { return excl; }
   __device__ __forceinline__ const float *color() const { return L; }
 
-  // Vertex after a geometry hit: the indirect sample and the child throughput.
-  __device__ __forceinline__ void setup(const dr_render_params &P, const Tables &tb, const Walk &w) {
-    h = w.h.id;
-    din = w.ray.d;
+  // The vertex's shading record (reference shaders; only the point for the others).
+  __device__ __forceinline__ RefHit hit_record(const dr_render_params &P, const Tables &tb,
+                                               const dr_material &M) const {
+    RefHit H;
+    H.p = hp;
+    if (is_reference_kind(M.kind)) {
+      const TexCtx T = tex_ctx(P);
+      H = ref_hit(tb.faces, h, hp, T, material_maps(T, M));
+    }
+    return H;
+  }
+  __device__ __forceinline__ V3 facing(const Tables &tb) const {
     const float4 sh = tb.shade[h];
-    mid = __float_as_int(sh.w);
-    const dr_material M = P.materials[mid];
-    const V3 du = normalized(din);
-    nu = facing_normal(v3(sh.x, sh.y, sh.z), du);
-    H.p = w.h.p;
-    const TexCtx T = tex_ctx(P);
-    if (is_reference_kind(M.kind)) H = ref_hit(tb.faces, h, w.h.p, T, material_maps(T, M));
-    has = false;
-    stype = 0;
-    float sw = 1.0f;
-    if (layer < P.max_bounces) {
-      float u[4];
-      rng4(P, pix, samp, (uint32_t)layer, 1, u);
-      bool alive = true;
-      if (layer >= P.rr_start) {
-        if (u[2] >= P.rr_survive) alive = false;
-        else sw = 1.0f / P.rr_survive;
-      }
-      if (alive) {
-        if (M.kind == DR_LAMBERT) { sdir = cosine_dir(nu, u[0], u[1]); has = true; }
-        else if (M.kind == DR_MIRROR || M.kind == DR_METAL) { sdir = reflect_unit(du, nu); has = true; }
-        else has = sample_ref(P, M, H, din, pix, samp, layer, u, sdir, stype);
-      }
-    }
-    if (has) {
-      float F[3];
-      if (is_reference_kind(M.kind)) {
-        const float white[3] = {1.0f, 1.0f, 1.0f};
-        reflect_ref(M, H, neg(sdir), white, neg(din), false, stype, T, F);
-      } else if (M.kind == DR_METAL) {
-        const float f5 = schlick5(nu, sdir);
-        for (int c = 0; c < 3; c++) F[c] = M.albedo[c] + (1.0f - M.albedo[c]) * f5;
-      } else {
-        for (int c = 0; c < 3; c++) F[c] = M.albedo[c];
-      }
-      for (int c = 0; c < 3; c++) bnext[c] = beta[c] * (F[c] * sw);
-    }
-    light = (M.kind == DR_MIRROR || M.kind == DR_METAL) ? P.nlights : -1;   // delta BSDFs take no direct light
-    lk = 0;
+    return facing_normal(v3(sh.x, sh.y, sh.z), normalized(din));
   }
   // Direct light of the current photon at the vertex.
-  __device__ __forceinline__ void add_direct(const dr_render_params &P) {
+  __device__ __forceinline__ void add_direct(const dr_render_params &P, const Tables &tb) {
     const dr_material M = P.materials[mid];
     if (M.kind == DR_LAMBERT) {
       const V3 wn = unit ? pd : normalized(pd);
-      const float cos_i = -dot(nu, wn);
+      const float cos_i = -dot(facing(tb), wn);
       if (cos_i > 0.0f) {
         const float k = cos_i * kInvPi;
         for (int c = 0; c < 3; c++) L[c] += beta[c] * ((M.albedo[c] * k) * pc[c]);
       }
     } else {
+      const RefHit H = hit_record(P, tb, M);
       float f[3];
       reflect_ref(M, H, pd, pc, neg(din), true, 0, tex_ctx(P), f);
       for (int c = 0; c < 3; c++) L[c] += beta[c] * f[c];
     }
   }
+  // The indirect sample of the vertex and the child layer's ray and throughput; false when
+  // the path ends here.
+  __device__ __forceinline__ bool next_layer(const dr_render_params &P, const Tables &tb) {
+    if (layer >= P.max_bounces) return false;
+    const dr_material M = P.materials[mid];
+    float u[4];
+    rng4(P, pix, samp, (uint32_t)layer, 1, u);
+    float sw = 1.0f;
+    if (layer >= P.rr_start) {
+      if (u[2] >= P.rr_survive) return false;
+      sw = 1.0f / P.rr_survive;
+    }
+    const RefHit H = hit_record(P, tb, M);
+    V3 sdir;
+    int stype = 0;
+    float F[3];
+    if (M.kind == DR_LAMBERT) {
+      sdir = cosine_dir(facing(tb), u[0], u[1]);
+      for (int c = 0; c < 3; c++) F[c] = M.albedo[c];
+    } else if (M.kind == DR_MIRROR || M.kind == DR_METAL) {
+      const V3 nu = facing(tb);
+      sdir = reflect_unit(normalized(din), nu);
+      if (M.kind == DR_METAL) {
+        const float f5 = schlick5(nu, sdir);
+        for (int c = 0; c < 3; c++) F[c] = M.albedo[c] + (1.0f - M.albedo[c]) * f5;
+      } else {
+        for (int c = 0; c < 3; c++) F[c] = M.albedo[c];
+      }
+    } else {
+      if (!sample_ref(P, M, H, din, pix, samp, layer, u, sdir, stype)) return false;
+      const float white[3] = {1.0f, 1.0f, 1.0f};
+      reflect_ref(M, H, neg(sdir), white, neg(din), false, stype, tex_ctx(P), F);
+    }
+    for (int c = 0; c < 3; c++) beta[c] = beta[c] * (F[c] * sw);
+    o = hp;
+    d = sdir;
+    excl = h;
+    layer++;
+    phase = 0;
+    return true;
+  }
   // After the lane's cast ended; true when the path is done.
   __device__ __forceinline__ bool shade(const dr_render_params &P, const Tables &tb, const Walk &w) {
     if (phase == 0) {
       if (!w.hit) return true;                        // the layer ray missed (:418-421)
-      setup(P, tb, w);
+      h = w.h.id;
+      hp = w.h.p;
+      din = w.ray.d;
+      mid = __float_as_int(tb.shade[h].w);
+      const int kind = P.materials[mid].kind;
+      light = (kind == DR_MIRROR || kind == DR_METAL) ? P.nlights : -1;   // delta BSDFs take no direct light
+      lk = 0;
     } else if (w.hit) {
-      const V3 e = w.h.p - H.p;
+      const V3 e = w.h.p - hp;
       if (w.h.id == h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps)) {
-        add_direct(P);
+        add_direct(P, tb);
       } else {
         const int hb = w.h.id;
         const dr_material Mb = P.materials[__float_as_int(tb.shade[hb].w)];
@@ -300,11 +319,13 @@ struct RefPath {
       lk--;
       V3 q, wv;
       bool u1;
-      if (!photon_ref(P, P.lights[light], light, lk, pix, samp, layer, H.p, q, wv, pc, u1)) continue;
+      float c3[3];
+      if (!photon_ref(P, P.lights[light], light, lk, pix, samp, layer, hp, q, wv, c3, u1)) continue;
       pd = wv;
       unit = u1;
+      for (int c = 0; c < 3; c++) pc[c] = c3[c];
       if (!light_casts_shadows(P.lights[light])) {
-        add_direct(P);
+        add_direct(P, tb);
         continue;
       }
       o = q;
@@ -314,14 +335,7 @@ struct RefPath {
       phase = 1;
       return false;
     }
-    if (!has) return true;
-    for (int c = 0; c < 3; c++) beta[c] = bnext[c];
-    o = H.p;
-    d = sdir;
-    excl = h;
-    layer++;
-    phase = 0;
-    return false;
+    return !next_layer(P, tb);
   }
 };
 
@@ -335,14 +349,36 @@ struct BlockLds {
   unsigned long long *best;   // this wave's leaf-test slots (walk_leaf_wave)
 };
 
-// Renders framebuffer block `block` of one request (all lanes of the workgroup call it).
+// Framebuffer pixel `lp` of block `block`: the colour of this iteration, set (iteration 1)
+// or blended with weight 1/iteration (DumbRenderer.cu:64, BlockBasedFrameBuffer.impl.cuh:125-156).
+__device__ __forceinline__ void store_pixel(const dr_render_params &P, float *__restrict__ frame, int block, int lp,
+                                            const float a[3]) {
+  int x, y;
+  if (!block_pixel(P.width, P.height, block, lp, x, y)) return;
+  float4 *f = (float4 *)frame + (size_t)kBlockPixels * block + lp;
+  const float4 col = make_float4(a[0], a[1], a[2], 1.0f);
+  const float amount = (P.iteration <= 1) ? 1.0f : (1.0f / ((float)P.iteration));
+  if (amount >= 1.0f) *f = col;
+  else {
+    const float4 old = *f;
+    const float ia = (1.0f - amount);
+    *f = make_float4((col.x * amount) + (old.x * ia), (col.y * amount) + (old.y * ia),
+                     (col.z * amount) + (old.z * ia), (col.w * amount) + (old.w * ia));
+  }
+}
+
+// Renders framebuffer blocks [block, block + nb) of one request (all lanes of the
+// workgroup call it).  nb > 1 only when every sample of the nb blocks fits one round
+// of the LDS staging (nb * samples per pixel <= kRoundSamples): at low sample counts
+// one block has fewer paths than the workgroup has lanes.
 // refill_min: an idle lane takes a new job once at least this many lanes of its wave
 // are idle (or the whole wave is).
 template <class PathT>
 __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb, const dr_render_params &P,
-                                             float *__restrict__ frame, int block, const BlockLds &lds,
+                                             float *__restrict__ frame, int block, int nb, const BlockLds &lds,
                                              int refill_min) {
   const int spp = P.fsaa_x * P.fsaa_y;
+  const int npix = kBlockPixels * nb;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
   const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
   const bool clip = P.cull_backfaces != 0;
@@ -352,8 +388,8 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
   for (int t = threadIdx.x; t < kBlockPixels * 3; t += kRB) lds.acc[t] = 0.0f;
 
   for (int s_base = 0; s_base < spp; s_base += kRoundSamples) {
-    const int ns = min(kRoundSamples, spp - s_base);
-    const int njobs = kBlockPixels * ns;       // job j: local pixel j / ns, sample s_base + j % ns
+    const int ns = min(kRoundSamples / nb, spp - s_base);
+    const int njobs = npix * ns;       // job j: pixel j / ns of the run, sample s_base + j % ns
     if (threadIdx.x == 0) *lds.next_job = 0;
     __syncthreads();
     PathT ps;
@@ -383,8 +419,10 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
           const int slot = base + __popcll(wmask & below);
           if (slot < njobs) {
             job = slot;
-            const int lp = slot / ns, s = s_base + (slot - lp * ns);
-            if (!ps.start(P, block, lp, s, samp0)) {
+            const int rp = slot / ns, s = s_base + (slot - rp * ns);
+            const bool ok = (nb == 1) ? ps.start(P, block, rp, s, samp0)
+                                      : ps.start(P, block + rp / kBlockPixels, rp % kBlockPixels, s, samp0);
+            if (!ok) {
               const float zero[3] = {0.0f, 0.0f, 0.0f};
               finish(zero);
             } else {
@@ -416,42 +454,40 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
       }
     }
     __syncthreads();
-    if (threadIdx.x < kBlockPixels) {
-      for (int s = 0; s < ns; s++)
-        for (int c = 0; c < 3; c++) lds.acc[3 * threadIdx.x + c] += samp_col[3 * (threadIdx.x * ns + s) + c] * spw;
+    if (nb == 1) {
+      if (threadIdx.x < kBlockPixels) {
+        for (int s = 0; s < ns; s++)
+          for (int c = 0; c < 3; c++) lds.acc[3 * threadIdx.x + c] += samp_col[3 * (threadIdx.x * ns + s) + c] * spw;
+      }
+    } else {
+      // the only round: the pixel sums (same order as the accumulation above) go out at once
+      for (int rp = threadIdx.x; rp < npix; rp += kRB) {
+        float a[3] = {0.0f, 0.0f, 0.0f};
+        for (int s = 0; s < ns; s++)
+          for (int c = 0; c < 3; c++) a[c] += samp_col[3 * (rp * ns + s) + c] * spw;
+        store_pixel(P, frame, block + rp / kBlockPixels, rp % kBlockPixels, a);
+      }
     }
     __syncthreads();
   }
-  if (threadIdx.x < kBlockPixels) {
-    int x, y;
-    if (block_pixel(P.width, P.height, block, threadIdx.x, x, y)) {
-      float4 *f = (float4 *)frame + (size_t)kBlockPixels * block + threadIdx.x;
-      const float *a = lds.acc + 3 * threadIdx.x;
-      const float4 col = make_float4(a[0], a[1], a[2], 1.0f);
-      const float amount = (P.iteration <= 1) ? 1.0f : (1.0f / ((float)P.iteration));
-      if (amount >= 1.0f) *f = col;
-      else {
-        const float4 old = *f;
-        const float ia = (1.0f - amount);
-        *f = make_float4((col.x * amount) + (old.x * ia), (col.y * amount) + (old.y * ia),
-                         (col.z * amount) + (old.z * ia), (col.w * amount) + (old.w * ia));
-      }
-    }
-  }
+  if (nb == 1 && threadIdx.x < kBlockPixels) store_pixel(P, frame, block, threadIdx.x, lds.acc + 3 * threadIdx.x);
   __syncthreads();
 }
 
-template <class PathT>
+// kGrouped: workgroups take runs of `group` framebuffer blocks (low sample counts);
+// otherwise one block each, with the run length fixed at 1 at compile time.
+template <class PathT, bool kGrouped>
 __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables tb, dr_render_params P,
                                                                float *__restrict__ frame, int start_block,
-                                                               int levels) {
+                                                               int end_block, int group, int levels) {
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];
   __shared__ int next_job;
   __shared__ unsigned long long best[kRB];
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
                       best + (threadIdx.x & ~63u)};
-  render_block<PathT>(T, tb, P, frame, start_block + blockIdx.x, L, 1);
+  const int first = start_block + (int)blockIdx.x * group;
+  render_block<PathT>(T, tb, P, frame, first, kGrouped ? min(group, end_block - first) : 1, L, 1);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -640,7 +676,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
     const int4 w = items[it];
     const DevRequest &R = reqs[w.x];
     const Tables tb = {R.shade, nullptr};
-    for (int b = w.y; b < w.y + w.z; b++) render_block<BenchPath>(R.T, tb, R.P, R.frame, b, L, refill_min);
+    for (int b = w.y; b < w.y + w.z; b++) render_block<BenchPath>(R.T, tb, R.P, R.frame, b, 1, L, refill_min);
   }
 }
 
@@ -663,6 +699,16 @@ static int check_params(dr_scene s, const dr_render_params *p) {
   return DR_OK;
 }
 
+// Framebuffer blocks per workgroup: at low sample counts a block has fewer paths than a
+// workgroup has lanes, so a workgroup takes a run of blocks (within one round of the
+// sample staging) while the launch keeps at least kRenderGroupsPerCu workgroups per CU.
+constexpr int kMaxRenderGroup = 8;
+constexpr int kRenderGroupsPerCu = 4;
+static int render_group(int spp, int nblocks, int num_cu) {
+  const int fill = nblocks / (std::max(num_cu, 1) * kRenderGroupsPerCu);
+  return std::max(1, std::min({kMaxRenderGroup, kRoundSamples / spp, fill}));
+}
+
 static int block_count(const dr_render_params *p) { return ((p->width + 7) / 8) * ((p->height + 15) / 16); }
 
 }  // namespace dr
@@ -677,12 +723,13 @@ extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *fr
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + (size_t)dr::kBlockPixels * dr::kRoundSamples * 3) * 4;
   const dr::Tables tb = {(const float4 *)s->shade.p, (const float *)s->faces.p};
-  if (p->plugins == DR_PLUGINS_REFERENCE)
-    hipLaunchKernelGGL(dr::k_render<dr::RefPath>, dim3(end_block - start_block), dim3(dr::kRB), lds,
-                       (hipStream_t)stream, s->tree, tb, *p, frame, start_block, levels);
-  else
-    hipLaunchKernelGGL(dr::k_render<dr::BenchPath>, dim3(end_block - start_block), dim3(dr::kRB), lds,
-                       (hipStream_t)stream, s->tree, tb, *p, frame, start_block, levels);
+  const int group = dr::render_group(p->fsaa_x * p->fsaa_y, end_block - start_block, s->num_cu);
+  const int groups = (end_block - start_block + group - 1) / group;
+  auto kernel = (p->plugins == DR_PLUGINS_REFERENCE)
+                    ? (group > 1 ? dr::k_render<dr::RefPath, true> : dr::k_render<dr::RefPath, false>)
+                    : (group > 1 ? dr::k_render<dr::BenchPath, true> : dr::k_render<dr::BenchPath, false>);
+  hipLaunchKernelGGL(kernel, dim3(groups), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree, tb, *p, frame,
+                     start_block, end_block, group, levels);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
 }
