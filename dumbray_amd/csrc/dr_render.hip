@@ -24,6 +24,7 @@ constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per ro
 constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
 constexpr int kLeafMin = 16;           // pending leaves that trigger the (wave-shared) leaf tests
 constexpr int kShadeMin = 32;          // finished casts that trigger the path updates
+constexpr int kMaxRenderGroup = 8;     // framebuffer blocks one workgroup renders together (low spp)
 // Waves per SIMD the render kernels are compiled for: the register cap this sets
 // (168 VGPRs) costs a few spills but beats 2 waves per SIMD clearly.
 constexpr int kRenderWaves = 3;
@@ -701,8 +702,8 @@ static int check_params(dr_scene s, const dr_render_params *p) {
 
 // Framebuffer blocks per workgroup: at low sample counts a block has fewer paths than a
 // workgroup has lanes, so a workgroup takes a run of blocks (within one round of the
-// sample staging) while the launch keeps at least kRenderGroupsPerCu workgroups per CU.
-constexpr int kMaxRenderGroup = 8;
+// sample staging, at most kMaxRenderGroup) while the launch keeps at least
+// kRenderGroupsPerCu workgroups per CU.
 constexpr int kRenderGroupsPerCu = 4;
 static int render_group(int spp, int nblocks, int num_cu) {
   const int fill = nblocks / (std::max(num_cu, 1) * kRenderGroupsPerCu);
