@@ -77,6 +77,8 @@ int paeth(int a, int b, int c) {
   return c;
 }
 
+constexpr uint64_t kMaxPixels = 1ull << 28;   // 1 GiB of RGBA8
+
 struct Png {
   uint32_t w = 0, h = 0;
   int depth = 0, ctype = 0, interlace = 0;
@@ -89,9 +91,9 @@ bool parse(const std::vector<unsigned char> &f, Png &p, std::string &err) {
   size_t at = 8;
   bool ihdr = false;
   while (at + 12 <= f.size()) {
-    const uint32_t len = rd32(&f[at]);
+    const uint32_t len = rd32(f.data() + at);
     if (at + 12 + (size_t)len > f.size()) { err = "truncated chunk"; return false; }
-    const unsigned char *type = &f[at + 4], *data = &f[at + 8];
+    const unsigned char *type = f.data() + at + 4, *data = f.data() + at + 8;
     if (!std::memcmp(type, "IHDR", 4) && len >= 13) {
       p.w = rd32(data);
       p.h = rd32(data + 4);
@@ -111,9 +113,10 @@ bool parse(const std::vector<unsigned char> &f, Png &p, std::string &err) {
     at += 12 + (size_t)len;
   }
   if (!ihdr || p.w == 0 || p.h == 0 || p.w > (1u << 16) || p.h > (1u << 16)) { err = "bad or missing IHDR"; return false; }
+  if ((uint64_t)p.w * p.h > kMaxPixels) { err = "image larger than 2^28 pixels"; return false; }
   const int d = p.depth, c = p.ctype;
-  const bool ok = (c == 0 && (d == 1 || d == 2 || d == 4 || d == 8 || d == 16)) || (c == 3 && d <= 8 && (d & (d - 1)) == 0) ||
-                  ((c == 2 || c == 4 || c == 6) && (d == 8 || d == 16));
+  const bool ok = (c == 0 && (d == 1 || d == 2 || d == 4 || d == 8 || d == 16)) ||
+                  (c == 3 && (d == 1 || d == 2 || d == 4 || d == 8)) || ((c == 2 || c == 4 || c == 6) && (d == 8 || d == 16));
   if (!ok || p.interlace > 1) { err = "unsupported colour type / bit depth / interlace"; return false; }
   if (c == 3 && p.plte.empty()) { err = "palette image without PLTE"; return false; }
   return true;
@@ -200,7 +203,24 @@ bool decode(const std::vector<unsigned char> &file, std::vector<unsigned char> &
   w = p.w;
   h = p.h;
   const int bits = p.depth * channels(p.ctype);
-  // inflate
+  // the scanline bytes the image needs (filter byte + row per row, per Adam7 pass)
+  static const int X0[7] = {0, 4, 0, 2, 0, 1, 0}, Y0[7] = {0, 0, 4, 0, 2, 0, 1};
+  static const int DX[7] = {8, 8, 4, 4, 2, 2, 1}, DY[7] = {8, 8, 8, 4, 4, 2, 2};
+  auto pass_size = [&](int pass, uint32_t &pw, uint32_t &ph) {
+    pw = (w > (uint32_t)X0[pass]) ? (w - X0[pass] + DX[pass] - 1) / DX[pass] : 0;
+    ph = (h > (uint32_t)Y0[pass]) ? (h - Y0[pass] + DY[pass] - 1) / DY[pass] : 0;
+  };
+  size_t need = 0;
+  if (p.interlace == 0) {
+    need = (((size_t)w * bits + 7) / 8 + 1) * h;
+  } else {
+    for (int pass = 0; pass < 7; pass++) {
+      uint32_t pw, ph;
+      pass_size(pass, pw, ph);
+      if (pw && ph) need += (((size_t)pw * bits + 7) / 8 + 1) * ph;
+    }
+  }
+  // inflate (no further than the image needs)
   z_stream zs{};
   if (inflateInit(&zs) != Z_OK) { err = "zlib init"; return false; }
   std::vector<unsigned char> raw;
@@ -212,9 +232,10 @@ bool decode(const std::vector<unsigned char> &file, std::vector<unsigned char> &
     zs.next_out = buf;
     zs.avail_out = sizeof(buf);
     zr = inflate(&zs, Z_NO_FLUSH);
-    if (zr != Z_OK && zr != Z_STREAM_END) { inflateEnd(&zs); err = "corrupt image data"; return false; }
+    if (zr != Z_OK && zr != Z_STREAM_END && zr != Z_BUF_ERROR) { inflateEnd(&zs); err = "corrupt image data"; return false; }
     raw.insert(raw.end(), buf, buf + (sizeof(buf) - zs.avail_out));
-  } while (zr != Z_STREAM_END && zs.avail_in > 0);
+    if (zr == Z_BUF_ERROR) break;                 // no progress possible: the input ended
+  } while (zr != Z_STREAM_END && raw.size() < need);
   inflateEnd(&zs);
   rgba.assign((size_t)w * h * 4, 0);
   std::vector<unsigned char> rows;
@@ -227,12 +248,10 @@ bool decode(const std::vector<unsigned char> &file, std::vector<unsigned char> &
     return true;
   }
   // Adam7
-  static const int X0[7] = {0, 4, 0, 2, 0, 1, 0}, Y0[7] = {0, 0, 4, 0, 2, 0, 1};
-  static const int DX[7] = {8, 8, 4, 4, 2, 2, 1}, DY[7] = {8, 8, 8, 4, 4, 2, 2};
   size_t at = 0;
   for (int pass = 0; pass < 7; pass++) {
-    const uint32_t pw = (w > (uint32_t)X0[pass]) ? (w - X0[pass] + DX[pass] - 1) / DX[pass] : 0;
-    const uint32_t ph = (h > (uint32_t)Y0[pass]) ? (h - Y0[pass] + DY[pass] - 1) / DY[pass] : 0;
+    uint32_t pw, ph;
+    pass_size(pass, pw, ph);
     if (pw == 0 || ph == 0) continue;
     const size_t used = unfilter(raw.data() + at, raw.size() - at, pw, ph, bits, rows);
     if (!used) { err = "bad interlaced scanlines"; return false; }
@@ -269,22 +288,30 @@ bool png_decode_file(const std::string &path, std::vector<unsigned char> &rgba, 
 
 extern "C" int dr_png_info(const char *path, int *width, int *height) {
   if (!path) return dr::fail(DR_ERR_ARG, "dr_png_info: bad arguments");
-  std::vector<unsigned char> rgba;
-  uint32_t w = 0, h = 0;
-  std::string err;
-  if (!dr::png_decode_file(path, rgba, w, h, err)) return dr::fail(DR_ERR_ARG, "dr_png_info: " + err);
-  if (width) *width = (int)w;
-  if (height) *height = (int)h;
-  return DR_OK;
+  try {
+    std::vector<unsigned char> rgba;
+    uint32_t w = 0, h = 0;
+    std::string err;
+    if (!dr::png_decode_file(path, rgba, w, h, err)) return dr::fail(DR_ERR_ARG, "dr_png_info: " + err);
+    if (width) *width = (int)w;
+    if (height) *height = (int)h;
+    return DR_OK;
+  } catch (const std::bad_alloc &) {
+    return dr::fail(DR_ERR_ALLOC, "dr_png_info: out of host memory");
+  }
 }
 
 extern "C" int dr_png_read(const char *path, uint8_t *rgba, int width, int height) {
   if (!path || !rgba) return dr::fail(DR_ERR_ARG, "dr_png_read: bad arguments");
-  std::vector<unsigned char> px;
-  uint32_t w = 0, h = 0;
-  std::string err;
-  if (!dr::png_decode_file(path, px, w, h, err)) return dr::fail(DR_ERR_ARG, "dr_png_read: " + err);
-  if ((int)w != width || (int)h != height) return dr::fail(DR_ERR_ARG, "dr_png_read: size differs from dr_png_info's");
-  std::memcpy(rgba, px.data(), px.size());
-  return DR_OK;
+  try {
+    std::vector<unsigned char> px;
+    uint32_t w = 0, h = 0;
+    std::string err;
+    if (!dr::png_decode_file(path, px, w, h, err)) return dr::fail(DR_ERR_ARG, "dr_png_read: " + err);
+    if ((int)w != width || (int)h != height) return dr::fail(DR_ERR_ARG, "dr_png_read: size differs from dr_png_info's");
+    std::memcpy(rgba, px.data(), px.size());
+    return DR_OK;
+  } catch (const std::bad_alloc &) {
+    return dr::fail(DR_ERR_ALLOC, "dr_png_read: out of host memory");
+  }
 }

@@ -1203,13 +1203,9 @@ struct dr_scene_file_s {
 };
 
 namespace {
-int finish_load(dr_scene_file_s *f, bool ok, dr_scene_file *out) {
-  if (!ok) {
-    const std::string msg = f->err.empty() ? std::string("scene file could not be loaded") : f->err;
-    delete f;
-    return dr::fail(DR_ERR_ARG, msg);
-  }
-  *out = f;
+int finish_load(std::unique_ptr<dr_scene_file_s> f, bool ok, dr_scene_file *out) {
+  if (!ok) return dr::fail(DR_ERR_ARG, f->err.empty() ? std::string("scene file could not be loaded") : f->err);
+  *out = f.release();
   return DR_OK;
 }
 }  // namespace
@@ -1220,9 +1216,9 @@ int dr_scene_file_load(const char *path, dr_scene_file *out) {
   if (!path || !out) return dr::fail(DR_ERR_ARG, "dr_scene_file_load: bad arguments");
   *out = nullptr;
   try {
-    auto *f = new dr_scene_file_s();
+    auto f = std::make_unique<dr_scene_file_s>();   // freed here if loading throws
     const bool ok = f->from_file(path, 0);
-    return finish_load(f, ok, out);
+    return finish_load(std::move(f), ok, out);
   } catch (const std::bad_alloc &) {
     return dr::fail(DR_ERR_ALLOC, "dr_scene_file_load: out of host memory");
   } catch (const std::exception &e) {
@@ -1234,9 +1230,13 @@ int dr_scene_file_parse(const char *text, const char *base_dir, dr_scene_file *o
   if (!text || !out) return dr::fail(DR_ERR_ARG, "dr_scene_file_parse: bad arguments");
   *out = nullptr;
   try {
-    auto *f = new dr_scene_file_s();
-    const bool ok = f->from_text(text, base_dir ? base_dir : "", 0);
-    return finish_load(f, ok, out);
+    // names are appended to the directory as the reference appends them to its source
+    // path, which ends in a separator
+    std::string dir = base_dir ? base_dir : "";
+    if (!dir.empty() && dir.back() != '/' && dir.back() != '\\') dir += '/';
+    auto f = std::make_unique<dr_scene_file_s>();
+    const bool ok = f->from_text(text, dir, 0);
+    return finish_load(std::move(f), ok, out);
   } catch (const std::bad_alloc &) {
     return dr::fail(DR_ERR_ALLOC, "dr_scene_file_parse: out of host memory");
   } catch (const std::exception &e) {

@@ -147,3 +147,43 @@ def test_png_reader_rejects_garbage(tmp_path):
     p.write_bytes(b"not a png")
     w, h = ctypes.c_int(), ctypes.c_int()
     assert D.lib().dr_png_info(str(p).encode(), ctypes.byref(w), ctypes.byref(h)) != 0
+
+
+@pytest.mark.parametrize("case", ["palette_depth0", "grey_depth3", "too_many_pixels", "empty_idat", "short_rows",
+                                  "zero_length_tail"])
+def test_png_reader_rejects_malformed_headers_and_data(tmp_path, case):
+    """Malformed inputs found by fuzzing the reader: each is an error, not a crash."""
+    import zlib
+    rows = b"\x00" + b"\x01\x02"           # one 2-pixel row of palette indices (depth 8)
+    plte = _chunk(b"PLTE", bytes(range(9)))
+    sig, end = b"\x89PNG\r\n\x1a\n", _chunk(b"IEND", b"")
+
+    def hdr(w, h, depth, ctype):
+        import struct
+        return _chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
+    data = {
+        "palette_depth0": sig + hdr(2, 1, 0, 3) + plte + _chunk(b"IDAT", zlib.compress(rows)) + end,
+        "grey_depth3": sig + hdr(2, 1, 3, 0) + _chunk(b"IDAT", zlib.compress(rows)) + end,
+        "too_many_pixels": sig + hdr(65536, 65536, 8, 0) + _chunk(b"IDAT", zlib.compress(b"")) + end,
+        "empty_idat": sig + hdr(2, 1, 8, 3) + plte + _chunk(b"IDAT", b"") + end,
+        "short_rows": sig + hdr(2, 4, 8, 3) + plte + _chunk(b"IDAT", zlib.compress(rows)) + end,
+        "zero_length_tail": sig + hdr(2, 1, 8, 7) + _chunk(b"tEXt", b""),
+    }[case]
+    p = tmp_path / "m.png"
+    p.write_bytes(data)
+    w, h = ctypes.c_int(), ctypes.c_int()
+    assert D.lib().dr_png_info(str(p).encode(), ctypes.byref(w), ctypes.byref(h)) != 0
+    assert D.lib().dr_last_error()
+
+
+def test_png_reader_large_compressible_image(tmp_path):
+    """A 1500 x 1100 image of one colour with a few odd pixels (6.6 MB of scanlines from a
+    few KB of deflate data) decodes to Pillow's pixels."""
+    from PIL import Image
+    img = np.zeros((1100, 1500, 4), np.uint8)
+    img[...] = (10, 200, 30, 255)
+    img[7, 3] = (1, 2, 3, 4)
+    img[1099, 1499] = (250, 251, 252, 253)
+    path = str(tmp_path / "big.png")
+    Image.fromarray(img, "RGBA").save(path)
+    np.testing.assert_array_equal(_read(path), img)

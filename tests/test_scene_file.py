@@ -534,6 +534,17 @@ def test_textured_scene_matches_oracle(tmp_path):
     np.testing.assert_array_equal(ctx.uv[0], [0, 0, 0, 1, 1, 1])
 
 
+def test_parse_base_dir_with_or_without_separator(tmp_path):
+    """dr_scene_file_parse resolves relative OBJ and PNG names against base_dir whether or
+    not it ends in a separator; the result equals loading the file itself."""
+    path = write_textured_scene(tmp_path)
+    text = open(path).read()
+    want = D.DumbRenderContext.fromFile(path)
+    for base in (str(tmp_path), str(tmp_path) + "/"):
+        got = D.DumbRenderContext.fromText(text, base)
+        assert same(got.tris, want.tris) and same(got.texels, want.texels) and same(got.uv, want.uv)
+
+
 @pytest.mark.parametrize("entry", ['"diffuse_texture": {"png": "tex/none.png"}',
                                    '"diffuse_texture": {"png": "tex/checker.png", "filtering": "cubic"}',
                                    '"diffuse_texture": {"file": "tex/checker.png"}',
