@@ -650,15 +650,18 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
 struct DevRequest {
   DevTree T;
   const float4 *shade;
+  const float *faces;            // shading records of the reference plugins
   dr_render_params P;
   float *frame;
 };
 
-__global__ void k_zero(int *counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+__global__ void k_zero(int *counter, int n) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < n) counter[threadIdx.x] = 0;
 }
 
-// Persistent workgroups drain the queue of (request, first block, block count) items.
+// Persistent workgroups drain the queue of (request, first block, block count) items
+// of the requests of one plugin set.
+template <class PathT>
 __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *__restrict__ reqs, const int4 *__restrict__ items,
                                                 int nitems, int *__restrict__ counter, int levels, int refill_min) {
   extern __shared__ uint32_t lds[];
@@ -676,8 +679,8 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
     if (it >= nitems) break;
     const int4 w = items[it];
     const DevRequest &R = reqs[w.x];
-    const Tables tb = {R.shade, nullptr};
-    for (int b = w.y; b < w.y + w.z; b++) render_block<BenchPath>(R.T, tb, R.P, R.frame, b, 1, L, refill_min);
+    const Tables tb = {R.shade, R.faces};
+    for (int b = w.y; b < w.y + w.z; b++) render_block<PathT>(R.T, tb, R.P, R.frame, b, 1, L, refill_min);
   }
 }
 
@@ -769,9 +772,10 @@ extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *d
 
 // ------------------------------------------------------------------ batch host side
 struct dr_batch_s {
-  int n = 0, nitems = 0, levels = 1, num_cu = 1;
+  int n = 0, levels = 1, num_cu = 1;
+  int nitems[2] = {0, 0};          // work items of the benchmark / reference plugin requests
   dr_batch_tuning tuning{1, 2, 1};
-  dr::DevBuf reqs, items, counter;
+  dr::DevBuf reqs, items[2], counter;
   std::vector<dr::DevRequest> host_reqs;
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -784,11 +788,13 @@ static size_t batch_lds(int levels) {
   return ((size_t)levels * kStackWordsPerLevel * kRB + (size_t)kBlockPixels * kRoundSamples * 3) * 4;
 }
 
-// Queue of (request, first block, count), heaviest per-block cost first.
-static std::vector<int4> make_items(const std::vector<DevRequest> &R, int tile) {
+// Queue of (request, first block, count) of the requests of plugin set `set` (0 =
+// benchmark, 1 = reference), heaviest per-block cost first.
+static std::vector<int4> make_items(const std::vector<DevRequest> &R, int tile, int set) {
   std::vector<std::pair<double, int4>> v;
   for (int r = 0; r < (int)R.size(); r++) {
     const dr_render_params &P = R[r].P;
+    if ((P.plugins == DR_PLUGINS_REFERENCE) != (set == 1)) continue;
     const int nb = block_count(&P);
     const double cost = (double)P.fsaa_x * P.fsaa_y * (1.0 + P.max_bounces) * (1.0 + P.nlights);
     for (int b = 0; b < nb; b += tile) v.push_back({cost * std::min(tile, nb - b), make_int4(r, b, std::min(tile, nb - b), 0)});
@@ -802,32 +808,44 @@ static std::vector<int4> make_items(const std::vector<DevRequest> &R, int tile) 
   return out;
 }
 
-static int launch_batch(const DevRequest *reqs_dev, const int4 *items_dev, int nitems, int *counter, int levels,
-                        const dr_batch_tuning &t, int num_cu, hipStream_t stream) {
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, counter);
-  hipLaunchKernelGGL(k_batch, dim3(num_cu * t.groups_per_cu), dim3(kRB), batch_lds(levels), stream, reqs_dev,
-                     items_dev, nitems, counter, levels, t.refill_min);
+// Queue resets, then one persistent kernel per plugin set that has requests (the
+// reference estimator's kernel after the benchmark one, on the same stream).
+static int launch_batch(const DevRequest *reqs_dev, const int4 *const items_dev[2], const int nitems[2], int *counter,
+                        int levels, const dr_batch_tuning &t, int num_cu, hipStream_t stream) {
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, counter, 2);
+  const dim3 grid(num_cu * t.groups_per_cu);
+  if (nitems[0] > 0)
+    hipLaunchKernelGGL(k_batch<BenchPath>, grid, dim3(kRB), batch_lds(levels), stream, reqs_dev, items_dev[0],
+                       nitems[0], counter, levels, t.refill_min);
+  if (nitems[1] > 0)
+    hipLaunchKernelGGL(k_batch<RefPath>, grid, dim3(kRB), batch_lds(levels), stream, reqs_dev, items_dev[1],
+                       nitems[1], counter + 1, levels, t.refill_min);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
 }
 
 // Times one tuning on the proxy (every request at 1x1 spp, iteration 1, scratch frames).
 static int time_tuning(const std::vector<DevRequest> &proxy, const dr_batch_tuning &t, int levels, int num_cu,
-                       hipStream_t stream, DevBuf &reqs_dev, DevBuf &items_dev, int *counter, float &ms) {
-  const std::vector<int4> items = make_items(proxy, t.tile_blocks);
-  dev_free(items_dev);
-  int rc = dev_upload(items_dev, items.data(), items.size() * sizeof(int4));
+                       hipStream_t stream, DevBuf &reqs_dev, DevBuf items_dev[2], int *counter, float &ms) {
+  int nitems[2];
+  const int4 *items_p[2];
+  int rc = DR_OK;
+  for (int set = 0; set < 2 && rc == DR_OK; set++) {
+    const std::vector<int4> items = make_items(proxy, t.tile_blocks, set);
+    nitems[set] = (int)items.size();
+    dev_free(items_dev[set]);
+    rc = dev_upload(items_dev[set], items.data(), items.size() * sizeof(int4));
+    items_p[set] = (const int4 *)items_dev[set].p;
+  }
   if (rc != DR_OK) return rc;
   hipEvent_t e0, e1;
   DR_HIP_CHECK(hipEventCreate(&e0));
   DR_HIP_CHECK(hipEventCreate(&e1));
-  rc = launch_batch((const DevRequest *)reqs_dev.p, (const int4 *)items_dev.p, (int)items.size(), counter, levels, t,
-                    num_cu, stream);
+  rc = launch_batch((const DevRequest *)reqs_dev.p, items_p, nitems, counter, levels, t, num_cu, stream);
   if (rc == DR_OK) {
     DR_HIP_CHECK(hipEventRecord(e0, stream));
     for (int k = 0; k < 2 && rc == DR_OK; k++)
-      rc = launch_batch((const DevRequest *)reqs_dev.p, (const int4 *)items_dev.p, (int)items.size(), counter,
-                        levels, t, num_cu, stream);
+      rc = launch_batch((const DevRequest *)reqs_dev.p, items_p, nitems, counter, levels, t, num_cu, stream);
     DR_HIP_CHECK(hipEventRecord(e1, stream));
     DR_HIP_CHECK(hipEventSynchronize(e1));
     DR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
@@ -850,10 +868,10 @@ static int autotune(const std::vector<DevRequest> &R, int levels, int num_cu, hi
     if (e != hipSuccess) { frames[r].p = nullptr; rc = fail(DR_ERR_ALLOC, "dr_batch_create: scratch frames"); }
     proxy[r].frame = (float *)frames[r].p;
   }
-  DevBuf reqs_dev, items_dev, counter;
+  DevBuf reqs_dev, items_dev[2], counter;
   if (rc == DR_OK) rc = dev_upload(reqs_dev, proxy.data(), proxy.size() * sizeof(DevRequest));
-  int zero = 0;
-  if (rc == DR_OK) rc = dev_upload(counter, &zero, sizeof(int));
+  const int zero[2] = {0, 0};
+  if (rc == DR_OK) rc = dev_upload(counter, zero, sizeof(zero));
   best = dr_batch_tuning{1, 2, 1};
   float best_ms = 0.0f;
   if (rc == DR_OK) rc = time_tuning(proxy, best, levels, num_cu, stream, reqs_dev, items_dev, (int *)counter.p, best_ms);
@@ -874,7 +892,8 @@ static int autotune(const std::vector<DevRequest> &R, int levels, int num_cu, hi
     }
   }
   dev_free(reqs_dev);
-  dev_free(items_dev);
+  dev_free(items_dev[0]);
+  dev_free(items_dev[1]);
   dev_free(counter);
   for (auto &f : frames) dev_free(f);
   return rc;
@@ -891,14 +910,11 @@ extern "C" int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tun
   for (int r = 0; r < n && rc == DR_OK; r++) {
     rc = dr::check_params(reqs[r].scene, &reqs[r].params);
     if (rc == DR_OK && !reqs[r].frame) rc = dr::fail(DR_ERR_ARG, "dr_batch_create: null frame");
-    if (rc == DR_OK && reqs[r].params.plugins != DR_PLUGINS_BENCHMARK)
-      rc = dr::fail(DR_ERR_ARG, "dr_batch_create: batches render the benchmark plugin set only");
-    if (rc == DR_OK && reqs[r].params.lens == DR_SPHERICAL_SEGMENT_LENS)
-      rc = dr::fail(DR_ERR_ARG, "dr_batch_create: batches do not take the spherical segment lens");
     if (rc != DR_OK) break;
     dr::DevRequest d;
     d.T = reqs[r].scene->tree;
     d.shade = (const float4 *)reqs[r].scene->shade.p;
+    d.faces = (const float *)reqs[r].scene->faces.p;
     d.P = reqs[r].params;
     d.frame = reqs[r].frame;
     b->host_reqs.push_back(d);
@@ -918,18 +934,21 @@ extern "C" int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tun
   if (rc == DR_OK && (b->tuning.tile_blocks < 1 || b->tuning.groups_per_cu < 1 || b->tuning.refill_min < 1))
     rc = dr::fail(DR_ERR_ARG, "dr_batch_create: tuning values must be >= 1");
   if (rc == DR_OK) {
-    const std::vector<int4> items = dr::make_items(b->host_reqs, b->tuning.tile_blocks);
-    b->nitems = (int)items.size();
-    int zero = 0;
     rc = dr::dev_upload(b->reqs, b->host_reqs.data(), b->host_reqs.size() * sizeof(dr::DevRequest));
-    if (rc == DR_OK) rc = dr::dev_upload(b->items, items.data(), items.size() * sizeof(int4));
-    if (rc == DR_OK) rc = dr::dev_upload(b->counter, &zero, sizeof(int));
+    for (int set = 0; set < 2 && rc == DR_OK; set++) {
+      const std::vector<int4> items = dr::make_items(b->host_reqs, b->tuning.tile_blocks, set);
+      b->nitems[set] = (int)items.size();
+      rc = dr::dev_upload(b->items[set], items.data(), items.size() * sizeof(int4));
+    }
+    const int zero[2] = {0, 0};
+    if (rc == DR_OK) rc = dr::dev_upload(b->counter, zero, sizeof(zero));
   }
   if (rc == DR_OK) {
     hipError_t e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      rc = dr::launch_batch((const dr::DevRequest *)b->reqs.p, (const int4 *)b->items.p, b->nitems, (int *)b->counter.p,
-                            b->levels, b->tuning, b->num_cu, b->cap_stream);
+      const int4 *items_p[2] = {(const int4 *)b->items[0].p, (const int4 *)b->items[1].p};
+      rc = dr::launch_batch((const dr::DevRequest *)b->reqs.p, items_p, b->nitems, (int *)b->counter.p, b->levels,
+                            b->tuning, b->num_cu, b->cap_stream);
       hipError_t e2 = hipStreamEndCapture(b->cap_stream, &b->graph);
       if (rc == DR_OK && e2 != hipSuccess) rc = dr::fail(DR_ERR_HIP, "dr_batch_create: graph capture");
       if (rc == DR_OK && hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
@@ -961,7 +980,8 @@ extern "C" void dr_batch_destroy(dr_batch b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   dr::dev_free(b->reqs);
-  dr::dev_free(b->items);
+  dr::dev_free(b->items[0]);
+  dr::dev_free(b->items[1]);
   dr::dev_free(b->counter);
   delete b;
 }

@@ -270,6 +270,8 @@ typedef struct dr_batch_s *dr_batch;
  * drained by persistent workgroups through one device atomic; the launch (queue
  * reset + kernel) is captured once in a hipGraph.  tuning == NULL runs the autotuner
  * (timed proxy renders into scratch frames; request frames are not touched).
+ * Requests of both plugin sets and every lens may be mixed: each plugin set has its own
+ * queue and persistent kernel, launched one after the other in the same graph.
  * Every request's frame equals what dr_render_blocks would produce for it. */
 int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tuning *tuning, dr_batch *out);
 /* Replays the captured graph on `stream`. */
