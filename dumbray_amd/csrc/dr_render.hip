@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
     Walk w;
     w.state = kWalkDone;
     int K = 0;                      // vertices recorded for the current path
-    float wdl[3] = {0, 0, 0};       // spw * dL/dI of the current pixel
+    float wdl[3] = {0, 0, 0};       // spw * dL/dI (* lens filter) of the current pixel
     bool exhausted = false;
     auto finish = [&]() {
       float Ln[3] = {0, 0, 0};
@@ -586,7 +586,9 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
               st.job = -1;
             } else {
               const float *g = dl + 4 * ((size_t)kBlockPixels * block + lp);
-              for (int c = 0; c < 3; c++) wdl[c] = spw * g[c];
+              // SphericalSegmentLense::getPixelColor scales the sample colour by its filter
+              for (int c = 0; c < 3; c++)
+                wdl[c] = spw * g[c] * (P.lens == DR_SPHERICAL_SEGMENT_LENS ? P.lens_p[4 + c] : 1.0f);
               walk_begin(T, w, st.o, st.d, st.restrict_tri, stk, kRB, nullptr);
             }
           } else {
@@ -744,7 +746,6 @@ extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *d
   if (rc != DR_OK) return rc;
   if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return dr::fail(DR_ERR_ARG, "dr_backward: null buffers");
   if (p->plugins != DR_PLUGINS_BENCHMARK) return dr::fail(DR_ERR_ARG, "dr_backward: only the benchmark plugin set has an adjoint");
-  if (p->lens == DR_SPHERICAL_SEGMENT_LENS) return dr::fail(DR_ERR_ARG, "dr_backward: the spherical segment lens has no adjoint");
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const int maxv = p->max_bounces + 1;
   const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;

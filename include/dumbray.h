@@ -241,7 +241,8 @@ int dr_render_blocks(dr_scene s, const dr_render_params *p, float *frame, int st
  * between passes).  dl_dframe: device, same block layout as `frame` (alpha ignored).
  * Adds dL/dalbedo (Lambert albedo and mirror reflectance; device double[nmat*3]) and
  * dL/demission (device double[nlights*3]); the caller zeroes them.  Per-block
- * reductions in LDS precede one FP64 atomic per block and parameter. */
+ * reductions in LDS precede one FP64 atomic per block and parameter.  The benchmark
+ * plugin set only; every lens (the spherical segment lens's filter scales dL/dI). */
 int dr_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo, double *g_emission,
                 void *stream);
 

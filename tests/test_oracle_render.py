@@ -121,8 +121,15 @@ def test_fp32_close_to_fp64(oracle):
     assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, float(b.max()))
 
 
-def test_adjoint_matches_central_differences(oracle):
-    sc, R = _spheres_renderer(oracle, 24, 16, (2, 2), 3)
+@pytest.mark.parametrize("lens", ["stochastic", "spherical_segment"])
+def test_adjoint_matches_central_differences(oracle, lens):
+    kw = {}
+    if lens == "spherical_segment":       # the lens filter scales the colour, so each channel's gradient
+        from dumbray_amd import plugins as PL
+        cam = S.spheres_scene(8, seed=0).camera.as_array()
+        kw = dict(lens=2, lens_params=PL.spherical_segment_lens((70.0, 45.0), 12.0, 0.3, (1.0, 0.8, 0.6)),
+                  cam_rows=np.array(cam[3:12], np.float32))
+    sc, R = _spheres_renderer(oracle, 24, 16, (2, 2), 3, **kw)
     w = np.random.default_rng(1).uniform(-1, 1, (16, 24, 3)).astype(np.float32)
     loss = lambda: float(np.sum(R.render(fp64=True) * w))
     ga, ge = R.backward(w)

@@ -83,11 +83,19 @@ def test_progressive_blend_and_block_range(dev, oracle):
     assert np.all(blk[:3] == 0) and np.all(blk[7:] == 0) and blk[3:7].sum() > 0
 
 
-def test_backward_matches_fp64_oracle_and_fd(dev, oracle):
+@pytest.mark.parametrize("lens", ["stochastic", "spherical_segment"])
+def test_backward_matches_fp64_oracle_and_fd(dev, oracle, lens):
     import torch
+    from dumbray_amd import plugins as PL
     sc = S.spheres_scene(8, seed=0)
     W, H = 48, 32
-    g, o = _pair(oracle, sc, W, H, (2, 2), 4)
+    kw = {}
+    if lens == "spherical_segment":       # filter (1, 0.8, 0.6) scales each channel's gradient
+        kw = dict(lens=PL.SPHERICAL_SEGMENT_LENS, lens_params=PL.spherical_segment_lens((70.0, 45.0), 12.0, 0.3,
+                                                                                      (1.0, 0.8, 0.6)),
+                  cam_rows=np.array(sc.camera.as_array()[3:12], np.float32))
+    print(f"[case] adjoint spheres {W}x{H} spp 2x2 depth 4 lens={lens}", flush=True)
+    g, o = _pair(oracle, sc, W, H, (2, 2), 4, **kw)
     rng = np.random.default_rng(3)
     wimg = rng.uniform(-1, 1, (H, W, 3)).astype(np.float32)
     f = D.BlockBasedFrameBuffer(W, H)
