@@ -129,17 +129,14 @@ __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g,
   hi = (head.y << 8) | ((head.x >> 8) & 0xffu);
 }
 
-// Initial traversal state.  Returns 0 = miss, 1 = stack ready, 2 = root is a leaf.
-__device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, uint32_t *stk, int stride, int &sp,
-                                          CastCounters *cc = nullptr) {
+// Initial traversal state.  Returns 0 = miss, 1 = traversal pending, 2 = root is a leaf.
+// The root's group (group 0, the root's children) is expanded by the first walk_step,
+// with the expansions of the other lanes, rather than here: a new ray is set up in the
+// divergent code that hands out work, where an expansion would run with few lanes.
+__device__ __forceinline__ int tree_begin(const DevTree &T, const TRay &r, int &sp) {
   if (T.root_is_leaf) return 2;
   if (!slab_hit(T.rsx, T.rsy, T.rsz, T.rex, T.rey, T.rez, r.o, r.inv)) return 0;
-  sp = 0;
-  uint32_t lo, hi;
-  group_entry(T.groups, 0, r, lo, hi);     // group 0 = the root's children
-  stk[0] = lo;
-  stk[stride] = hi;
-  if (cc) cc->expand++;
+  sp = -1;
   return 1;
 }
 
@@ -161,7 +158,7 @@ __device__ __forceinline__ void walk_begin(const DevTree &T, Walk &w, V3 o, V3 d
   set_prune(w.ray, T.margin);
   w.ex = exclude;
   w.hit = false;
-  const int st = tree_begin(T, w.ray, stk, stride, w.sp, cc);
+  const int st = tree_begin(T, w.ray, w.sp);
   if (st == 0) {
     w.state = kWalkDone;
   } else if (st == 2) {
@@ -173,37 +170,41 @@ __device__ __forceinline__ void walk_begin(const DevTree &T, Walk &w, V3 o, V3 d
 }
 
 // Pops one child: expands an interior child, or holds a leaf child as pending;
-// an exhausted stack ends the query with a miss.
+// an exhausted stack ends the query with a miss.  A walk's first step (sp == -1 after
+// walk_begin) expands the root's group.
 __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *stk, int stride, CastCounters *cc) {
-  uint32_t f = 0;
-  while (w.sp >= 0) {
-    f = stk[2 * w.sp * stride];
-    if (f & 0xffu) break;
-    --w.sp;
+  int child = 0;                      // group to expand; 0 (the root's) on a walk's first step
+  if (w.sp >= 0) {
+    uint32_t f = 0;
+    while (w.sp >= 0) {
+      f = stk[2 * w.sp * stride];
+      if (f & 0xffu) break;
+      --w.sp;
+    }
+    if (w.sp < 0) { w.state = kWalkDone; return; }
+    const int i = __builtin_ctz(f & 0xffu);
+    stk[2 * w.sp * stride] = f & ~(1u << i);
+    const int k = w.ray.canon ^ i;
+    const uint32_t hi = stk[(2 * w.sp + 1) * stride];
+    const uint32_t im = hi & 0xffu;
+    if (!((im >> k) & 1u)) {
+      const int *Gi = (const int *)T.groups;
+      const size_t base = kGroupWords * (size_t)(f >> 8);
+      w.lg = (int)(f >> 8);
+      w.lk = k;
+      w.llink = Gi[base + kGroupLink + k];
+      w.lcnt = Gi[base + kGroupCount + k];
+      w.state = kWalkLeaf;
+      return;
+    }
+    child = (int)(hi >> 8) + __builtin_popcount(im & ((1u << k) - 1u));
   }
-  if (w.sp < 0) { w.state = kWalkDone; return; }
-  const int i = __builtin_ctz(f & 0xffu);
-  stk[2 * w.sp * stride] = f & ~(1u << i);
-  const int k = w.ray.canon ^ i;
-  const uint32_t hi = stk[(2 * w.sp + 1) * stride];
-  const uint32_t im = hi & 0xffu;
-  if ((im >> k) & 1u) {
-    const int child = (int)(hi >> 8) + __builtin_popcount(im & ((1u << k) - 1u));
-    uint32_t lo2, hi2;
-    group_entry(T.groups, child, w.ray, lo2, hi2);
-    ++w.sp;
-    stk[2 * w.sp * stride] = lo2;
-    stk[(2 * w.sp + 1) * stride] = hi2;
-    if (cc) cc->expand++;
-  } else {
-    const int *Gi = (const int *)T.groups;
-    const size_t base = kGroupWords * (size_t)(f >> 8);
-    w.lg = (int)(f >> 8);
-    w.lk = k;
-    w.llink = Gi[base + kGroupLink + k];
-    w.lcnt = Gi[base + kGroupCount + k];
-    w.state = kWalkLeaf;
-  }
+  uint32_t lo2, hi2;
+  group_entry(T.groups, child, w.ray, lo2, hi2);
+  ++w.sp;
+  stk[2 * w.sp * stride] = lo2;
+  stk[(2 * w.sp + 1) * stride] = hi2;
+  if (cc) cc->expand++;
 }
 
 // castInLeaf for every pending lane of the wave at once, with all 64 lanes sharing
