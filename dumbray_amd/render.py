@@ -211,6 +211,45 @@ class DumbRenderer:
         self._iteration = k
         return True
 
+    # ------------------------------------------------ progress checkpoints (no counterpart in the
+    # reference, whose progress is only the in-memory iteration counter and frame, Renderer.cu:137-141)
+    def _progress_key(self):
+        """The settings that decide which samples iteration k draws and how they blend."""
+        rr = np.frombuffer(np.float32(self.rr_survive).tobytes(), np.int32)[0]
+        return np.array([self.width, self.height, self.fsaa_x, self.fsaa_y, self.max_bounces, self.rr_start, rr,
+                         int(self.ignore_backfaces), self.seed & 0xFFFFFFFF, (self.seed >> 32) & 0xFFFFFFFF, self.lens,
+                         self.plugins, self.boxing], np.int64)
+
+    def saveProgress(self, path):
+        """Writes the iteration count, the frame buffer (block-major RGBA fp32) and the settings
+        that decide the samples to an .npz file.  The scene, materials, lights and camera are
+        the caller's to restore.  Iterations are keyed by their number (Philox counters), so a
+        renderer that loads the file and iterates n more times ends on the frame of an
+        uninterrupted run."""
+        frame = self.getFrameBuffer()
+        if frame is None:
+            raise DumbRayError("saveProgress: no frame buffer")
+        np.savez(path, iteration=np.int64(self.iteration()), frame=frame.data.cpu().numpy(), key=self._progress_key())
+
+    def loadProgress(self, path, device="cuda"):
+        """Restores a saveProgress file into a new frame buffer of this renderer (which becomes
+        its frame buffer) and returns it.  Raises when the file's settings differ."""
+        import torch
+
+        with np.load(path, allow_pickle=False) as z:
+            it, data, key = int(z["iteration"]), z["frame"], z["key"]
+        w, h = int(key[0]), int(key[1])
+        self.width, self.height = w, h
+        if not np.array_equal(key, self._progress_key()):
+            raise DumbRayError("loadProgress: the file was written with other render settings")
+        frame = BlockBasedFrameBuffer(w, h, device)
+        if tuple(frame.data.shape) != tuple(data.shape):
+            raise DumbRayError("loadProgress: frame size differs from the settings")
+        frame.data.copy_(torch.from_numpy(data))
+        self.setFrameBuffer(frame)
+        self._iteration = it
+        return frame
+
     # parameters an optimiser may update in place
     @property
     def albedo(self):

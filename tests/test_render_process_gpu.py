@@ -169,3 +169,36 @@ def test_cpp_scene_file_workflow_matches_python(dev, tmp_path):
         f = g.render(f, k)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(np.asarray(Image.open(out)), D.images.frame_to_rgba8(f).cpu().numpy())
+
+
+def test_progress_checkpoint_resumes_bit_exactly(dev, tmp_path):
+    """saveProgress after 3 iterations, loadProgress into a fresh renderer, 2 more iterations:
+    the frame equals 5 uninterrupted iterations bit for bit (iterations are keyed by their
+    number); a renderer with other settings refuses the file."""
+    import torch
+    ctx = D.DumbRenderContext.fromFile(EXAMPLE)
+    tree = ctx.octree()
+    W, H = 40, 24
+    print(f"[case] progress checkpoint example scene {W}x{H}", flush=True)
+    a = ctx.renderer(W, H, tree=tree, samples_per_pixel=(2, 1))
+    fa = D.BlockBasedFrameBuffer(W, H)
+    a.setFrameBuffer(fa)
+    for _ in range(5):
+        assert a.iterate()
+    b = ctx.renderer(W, H, tree=tree, samples_per_pixel=(2, 1))
+    b.setFrameBuffer(D.BlockBasedFrameBuffer(W, H))
+    for _ in range(3):
+        assert b.iterate()
+    path = str(tmp_path / "progress.npz")
+    b.saveProgress(path)
+    c = ctx.renderer(W, H, tree=tree, samples_per_pixel=(2, 1))
+    fc = c.loadProgress(path)
+    assert c.iteration() == 3 and c.getFrameBuffer() is fc
+    for _ in range(2):
+        assert c.iterate()
+    torch.cuda.synchronize()
+    assert c.iteration() == 5
+    assert torch.equal(fa.data.view(torch.int32), fc.data.view(torch.int32))
+    other = ctx.renderer(W, H, tree=tree, samples_per_pixel=(1, 1))
+    with pytest.raises(D.DumbRayError):
+        other.loadProgress(path)
