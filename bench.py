@@ -287,8 +287,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (each GPU box has one MI355X); "
-              "multi-GPU sharding is described in DESIGN.md but not run.", file=sys.stderr)
+        print("bench.py: measures one MI355X only; runs on several GPUs (rays, tiles or requests "
+              "split across them) are not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import torch
