@@ -344,6 +344,9 @@ class DumbRenderer {
   ~DumbRenderer() {
     (void)hipFree(mats_);
     (void)hipFree(lights_);
+    (void)hipFree(maps_);
+    (void)hipFree(textures_);
+    (void)hipFree(texels_);
   }
   DumbRenderer(const DumbRenderer &) = delete;
   DumbRenderer &operator=(const DumbRenderer &) = delete;
@@ -358,8 +361,27 @@ class DumbRenderer {
     rrSurvive_ = survive;
   }
   void setSeed(uint64_t seed) { seed_ = seed; }
-  void setLens(int lens) { lens_ = lens; }   // DR_STOCHASTIC_LENS or DR_PERSPECTIVE_LENS
+  void setLens(int lens) { lens_ = lens; }   // DR_STOCHASTIC_LENS, DR_PERSPECTIVE_LENS or DR_SPHERICAL_SEGMENT_LENS
+  // Lenses::SphericalSegmentLense parameters (angles, focal distance, sensor size, filter)
+  // and the camera Transform's point-map rows; without rows, the camera's right, up, forward.
+  void setLensParams(const float lensP[8], const float *camRows = nullptr) {
+    for (int k = 0; k < 8; k++) lensP_[k] = lensP[k];
+    hasRows_ = camRows != nullptr;
+    if (camRows)
+      for (int k = 0; k < 9; k++) camRows_[k] = camRows[k];
+  }
+  // Texture tables (ColoredTexture maps of the materials that have them, the textures and
+  // their RGBA fp32 texels), copied to the device.
+  void setTextures(const std::vector<dr_material_maps> &maps, const std::vector<dr_texture> &textures,
+                   const std::vector<float> &texels) {
+    upload(maps_, maps);
+    upload(textures_, textures);
+    upload(texels_, texels);
+    nmaps_ = (int)maps.size();
+    ntextures_ = (int)textures.size();
+  }
   void setBoxingMode(BoxingMode mode) { boxing_ = mode; }
+  dr_scene deviceScene() const { return scene_; }
   dr_material *deviceMaterials() { return mats_; }
   dr_light *deviceLights() { return lights_; }
 
@@ -382,6 +404,13 @@ class DumbRenderer {
                            camera_.up.z,       camera_.forward.x,  camera_.forward.y,  camera_.forward.z,
                            camera_.focal};
     for (int k = 0; k < 13; k++) p.cam[k] = cam[k];
+    for (int k = 0; k < 9; k++) p.cam_rows[k] = hasRows_ ? camRows_[k] : cam[3 + k];
+    for (int k = 0; k < 8; k++) p.lens_p[k] = lensP_[k];
+    p.maps = maps_;
+    p.textures = textures_;
+    p.texels = texels_;
+    p.nmaps = nmaps_;
+    p.ntextures = ntextures_;
     p.materials = mats_;
     p.lights = lights_;
     p.nmat = nmat_;
@@ -430,10 +459,24 @@ class DumbRenderer {
   }
 
  private:
+  template <typename T, typename D>
+  static void upload(D *&dev, const std::vector<T> &v) {
+    (void)hipFree(dev);
+    dev = nullptr;
+    if (v.empty()) return;
+    hip_check(hipMalloc((void **)&dev, sizeof(T) * v.size()), "hipMalloc");
+    hip_check(hipMemcpy((void *)dev, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), "copy");
+  }
   dr_scene scene_;
   Camera camera_;
   dr_material *mats_ = nullptr;
   dr_light *lights_ = nullptr;
+  dr_material_maps *maps_ = nullptr;
+  dr_texture *textures_ = nullptr;
+  float *texels_ = nullptr;
+  int nmaps_ = 0, ntextures_ = 0;
+  float lensP_[8] = {0, 0, 0, 0, 0, 0, 0, 0}, camRows_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool hasRows_ = false;
   int nmat_, nlights_;
   int bounces_ = 2, fx_ = 1, fy_ = 1, rrStart_ = 1 << 30;
   float rrSurvive_ = 1.0f;
@@ -443,6 +486,43 @@ class DumbRenderer {
   int boxing_ = DR_BOXING_HEIGHT;
   BlockBuffer *frame_ = nullptr;
   int iteration_ = 0;
+};
+
+// Many renders in one persistent launch captured in a hipGraph (dr_batch_*), in place of
+// BlockRenderer's host-driven block batches (BlockRenderer.cu:74-97).  Requests of both
+// plugin sets and every lens may be mixed; each frame ends as renderBlocksGPU would leave it.
+class RenderBatch {
+ public:
+  struct Request {
+    DumbRenderer *renderer;
+    BlockBuffer *frame;
+    int iteration;
+    uint32_t request;        // Philox counter word 3
+  };
+  // tuning == nullptr runs the autotuner
+  explicit RenderBatch(const std::vector<Request> &requests, const dr_batch_tuning *tuning = nullptr) {
+    std::vector<dr_request> r(requests.size());
+    for (size_t i = 0; i < requests.size(); i++) {
+      int w, h;
+      requests[i].frame->getSize(&w, &h);
+      r[i].scene = requests[i].renderer->deviceScene();
+      r[i].params = requests[i].renderer->params(w, h, requests[i].iteration, requests[i].request);
+      r[i].frame = requests[i].frame->deviceData();
+    }
+    check(dr_batch_create(r.data(), (int)r.size(), tuning, &batch_), "dr_batch_create");
+  }
+  ~RenderBatch() { dr_batch_destroy(batch_); }
+  RenderBatch(const RenderBatch &) = delete;
+  RenderBatch &operator=(const RenderBatch &) = delete;
+  void run(hipStream_t stream = nullptr) { check(dr_batch_run(batch_, (void *)stream), "dr_batch_run"); }
+  dr_batch_tuning tuning() const {
+    dr_batch_tuning t{};
+    check(dr_batch_get_tuning(batch_, &t), "dr_batch_get_tuning");
+    return t;
+  }
+
+ private:
+  dr_batch batch_ = nullptr;
 };
 
 // Images::saveBufferPNG (Images.cu:22-36): the frame as an 8-bit RGBA PNG, each channel
@@ -620,14 +700,22 @@ class DumbRenderContext {
       if (errorStream) (*errorStream) << dr_last_error() << "\n";
       return false;
     }
-    int ntris = 0, nmat = 0, nlights = 0;
+    int ntris = 0, nmat = 0, nlights = 0, nt = 0, nm = 0;
+    int64_t ntx = 0;
     dr_scene_file_info(f, &settings_, &ntris, &nmat, &nlights);
-    std::vector<float> tris(9 * (size_t)ntris), nrm(9 * (size_t)ntris);
+    std::vector<float> tris(9 * (size_t)ntris), nrm(9 * (size_t)ntris), uv(6 * (size_t)ntris);
     std::vector<int32_t> mat(ntris);
     materials_.assign(nmat, dr_material{});
     lights_.assign(nlights, dr_light{});
     dr_scene_file_geometry(f, tris.data(), nrm.data(), mat.data());
+    if (ntris > 0) dr_scene_file_texcoords(f, uv.data());
     dr_scene_file_tables(f, materials_.data(), lights_.data());
+    // texture and map tables for the renderers
+    dr_scene_file_texture_info(f, &nt, &ntx, &nm);
+    textures_.assign(nt, dr_texture{});
+    texels_.assign(4 * (size_t)ntx, 0.0f);
+    maps_.assign(nm, dr_material_maps{});
+    dr_scene_file_textures(f, textures_.data(), texels_.data(), maps_.data());
     dr_scene_file_destroy(f);
     geometry_.reset();
     for (int i = 0; i < ntris; i++) {
@@ -640,7 +728,10 @@ class DumbRenderContext {
     }
     geometry_.build();
     normals_ = std::move(nrm);
-    if (ntris > 0) check(dr_scene_set_normals(geometry_.deviceScene(), normals_.data(), ntris), "dr_scene_set_normals");
+    if (ntris > 0) {
+      check(dr_scene_set_normals(geometry_.deviceScene(), normals_.data(), ntris), "dr_scene_set_normals");
+      check(dr_scene_set_texcoords(geometry_.deviceScene(), uv.data(), ntris), "dr_scene_set_texcoords");
+    }
     return true;
   }
   bool buildFromFile(const std::string &filename) { return buildFromFile<std::ostream>(filename, nullptr); }
@@ -659,7 +750,9 @@ class DumbRenderContext {
     auto *r = new DumbRenderer(geometry_, materials_, lights_, camera(), settings_.max_bounces, settings_.fsaa_x,
                                settings_.fsaa_y, settings_.ignore_backfaces != 0);
     r->setLens(settings_.lens);
+    r->setLensParams(settings_.lens_p, settings_.cam_rows);
     r->setBoxingMode((DumbRenderer::BoxingMode)settings_.boxing);
+    r->setTextures(maps_, textures_, texels_);
     return r;
   }
 
@@ -668,6 +761,9 @@ class DumbRenderContext {
   std::vector<dr_material> materials_;
   std::vector<dr_light> lights_;
   std::vector<float> normals_;
+  std::vector<dr_texture> textures_;
+  std::vector<float> texels_;
+  std::vector<dr_material_maps> maps_;
   dr_scene_file_settings settings_{};
 };
 

@@ -51,13 +51,21 @@ int main(int argc, char **argv) {
     if (!r.renderBlocksGPU(frame, 0, frame.getBlockCount(), 1)) throw Error("renderBlocksGPU");
     hip_check(hipDeviceSynchronize(), "sync");
     const std::vector<float> img = frame.toHostImage();
+    // the same request through RenderBatch (one persistent graph launch) gives the same frame
+    BlockBuffer bframe(W, H);
+    RenderBatch batch({{&r, &bframe, 1, 0}});
+    batch.run();
+    hip_check(hipDeviceSynchronize(), "sync");
+    const std::vector<float> bimg = bframe.toHostImage();
+    if (bimg.size() != img.size() || std::memcmp(bimg.data(), img.data(), img.size() * sizeof(float)) != 0)
+      throw Error("RenderBatch frame differs from renderBlocksGPU");
     FILE *o = fopen(argv[2], "wb");
     if (!o) { perror("out"); return 2; }
     const int32_t nodes = tree.getNodeCount();
     fwrite(&nodes, sizeof(nodes), 1, o);
     fwrite(img.data(), sizeof(float), img.size(), o);
     fclose(o);
-    printf("host_api_demo: %d tris, %d nodes, %dx%d rendered\n", ntris, nodes, W, H);
+    printf("host_api_demo: %d tris, %d nodes, %dx%d rendered, batch equal\n", ntris, nodes, W, H);
   } catch (const std::exception &e) {
     fprintf(stderr, "host_api_demo: %s\n", e.what());
     return 1;

@@ -39,6 +39,7 @@ def test_cpp_mirror_renders_like_python(dev, tmp_path):
     print("[case] C++ host mirror 50x34", flush=True)
     r = subprocess.run([EXE, scene_bin, out_bin], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    assert "batch equal" in r.stdout        # RenderBatch (C++ mirror) = renderBlocksGPU, checked in the demo
     raw = np.fromfile(out_bin, dtype=np.uint8)
     nodes = int(raw[:4].view(np.int32)[0])
     img_cpp = raw[4:].view(np.float32).reshape(H, W, 4)

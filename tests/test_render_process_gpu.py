@@ -151,18 +151,37 @@ def test_buffered_render_process_double_buffers(dev):
         _close(f.to_image()[..., :3].cpu().numpy(), ref)
 
 
-def test_cpp_scene_file_workflow_matches_python(dev, tmp_path):
+def _lens_scene(tmp_path):
+    """The example scene seen through the spherical segment lens."""
+    import re
+    text = open(EXAMPLE).read().replace('"files/cargo.obj"', '"%s"' % os.path.join(os.path.dirname(EXAMPLE), "files",
+                                                                                     "cargo.obj"))
+    text = re.sub(r'"lense"\s*:\s*\{[^}]*\}', '"lense": {"type": "spherical_segment", "angle": 40, "distance": 140, '
+                  '"sensor": 2.5, "filter": [1.0, 0.9, 0.7]}', text)
+    path = tmp_path / "lens.dumb"
+    path.write_text(text)
+    return str(path)
+
+
+@pytest.mark.parametrize("scene", ["example", "textured", "spherical_segment_lens"])
+def test_cpp_scene_file_workflow_matches_python(dev, tmp_path, scene):
+    """The C++ mirror's workflow writes the Python mirror's PNG for the example scene, a
+    textured scene file and the example scene through the spherical segment lens."""
     import torch
     from PIL import Image
+    from test_scene_file import write_textured_scene
 
     assert os.path.exists(DEMO), "run __graft_entry__.build() first"
+    path = {"example": lambda: EXAMPLE, "textured": lambda: write_textured_scene(tmp_path),
+            "spherical_segment_lens": lambda: _lens_scene(tmp_path)}[scene]()
     W, H, N = 64, 36, 3
     out = str(tmp_path / "cpp.png")
-    print("[case] C++ scene_file_demo 64x36 3 iterations", flush=True)
-    r = subprocess.run([DEMO, EXAMPLE, str(W), str(H), str(N), out], capture_output=True, text=True, timeout=120)
+    print(f"[case] C++ scene_file_demo {scene} 64x36 3 iterations", flush=True)
+    r = subprocess.run([DEMO, path, str(W), str(H), str(N), out], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
-    assert "triangles 1670" in r.stdout
-    ctx = D.DumbRenderContext.fromFile(EXAMPLE)
+    if scene == "example":
+        assert "triangles 1670" in r.stdout
+    ctx = D.DumbRenderContext.fromFile(path)
     g = ctx.renderer(W, H)
     f = None
     for k in range(1, N + 1):
