@@ -138,3 +138,33 @@ def test_gpu_cast_full_heightfield(dev, oracle):
         assert np.max(np.abs(p - g["point"][hit])) < 1e-2
         assert np.all(g["t"][~hit] == np.finfo(np.float32).max)
         print(f"[case] batch n={len(rays)} hit={hit.mean():.3f}", flush=True)
+
+
+def test_gpu_cast_concurrent_streams(dev, oracle):
+    """Casts in flight on several streams at once (each launch takes its own pair of
+    work counters from the scene's ring) give the oracle's results, on many launches in
+    a row (the ring wraps) and on batches smaller than one wave's range of rays."""
+    import torch
+    tris = S.heightfield(128, seed=3)
+    tree = D.Octree(16, 12)
+    tree.push(tris)
+    tree.build()
+    sc = tree.upload()
+    ot = oracle.Octree(tris, 16, 12)
+    sizes = [5, 200, 7000, 60000]
+    rays = [_rays(tris, n, seed=40 + k) for k, n in enumerate(sizes)]
+    refs = [ot.cast(r, clip=False) for r in rays]
+    dev_rays = [torch.from_numpy(r).cuda() for r in rays]
+    streams = [torch.cuda.Stream() for _ in sizes]
+    print(f"[case] concurrent casts on {len(streams)} streams, 300 rounds", flush=True)
+    torch.cuda.synchronize()
+    for rnd in range(300):                     # > the ring of 1024 counter pairs after all rounds
+        outs = []
+        for s, r in zip(streams, dev_rays):
+            with torch.cuda.stream(s):
+                outs.append(sc.cast(r, clip_backfaces=False, stream=s))
+        torch.cuda.synchronize()
+        if rnd % 50 == 0 or rnd == 299:
+            for o, ref in zip(outs, refs):
+                assert np.array_equal(o["id"].cpu().numpy(), ref["id"]), rnd
+                assert np.array_equal(o["t"].cpu().numpy().view(np.int32), ref["t"].view(np.int32)), rnd
