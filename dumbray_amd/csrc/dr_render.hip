@@ -479,22 +479,6 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
   __syncthreads();
 }
 
-// Workgroup index -> work index such that each XCD (the hardware hands workgroups to the
-// 8 XCDs in turn: w, w + 8, ... go to XCD w % 8) takes runs of kXcdChunk consecutive
-// blocks, the runs of the 8 XCDs interleaved over the frame: neighbouring blocks, which
-// see the same part of the scene, share an L2, while every XCD still gets work from the
-// whole frame (one contiguous eighth per XCD was 3.3x slower at 1080p: the image's
-// regions differ in cost).  A bijection on [0, n); the tail that does not fill 8 whole
-// runs keeps the identity.
-constexpr int kXcds = 8;
-constexpr int kXcdChunk = 16;
-__device__ __forceinline__ int xcd_major(int w, int n) {
-  const int span = kXcds * kXcdChunk, head = (n / span) * span;
-  if (w >= head) return w;
-  const int x = w % kXcds, i = w / kXcds;
-  return ((i / kXcdChunk) * kXcds + x) * kXcdChunk + i % kXcdChunk;
-}
-
 // kGrouped: workgroups take runs of `group` framebuffer blocks (low sample counts);
 // otherwise one block each, with the run length fixed at 1 at compile time.
 template <class PathT, bool kGrouped>
@@ -507,7 +491,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
   __shared__ unsigned long long best[kRB];
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
                       best + (threadIdx.x & ~63u)};
-  const int first = start_block + xcd_major(blockIdx.x, gridDim.x) * group;
+  const int first = start_block + (int)blockIdx.x * group;
   render_block<PathT>(T, tb, P, frame, first, kGrouped ? min(group, end_block - first) : 1, L, kRenderRefillMin);
 }
 
