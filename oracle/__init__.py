@@ -469,9 +469,12 @@ class Octree:
         self.h = lib().ora_octree_build(self.tris, int(self.tris.shape[0]), int(split_threshold), int(max_depth))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().ora_octree_free(self.h)
-            self.h = None
+        try:
+            if getattr(self, "h", None):
+                lib().ora_octree_free(self.h)
+                self.h = None
+        except Exception:            # interpreter shutdown: the module's globals may be gone
+            pass
 
     @property
     def node_count(self):
