@@ -33,10 +33,11 @@ B_RAY, B_OUT, B_BOX, B_REF, B_TRI = 24, 8, 28, 4, 36
 OPS_RAY, OPS_BOX, OPS_TRI = 6, 24, 155
 # Work of the kernel's traversal per ray (counted by dr_cast_stats on the same rays): a group
 # expansion reads 196 B (8 child boxes + the non-empty mask) and pops read 8 B (link, count); a leaf
-# reads its 24-B box; a plane test reads 32 B of the record, an inside test 32 B more.
+# reads its 24-B box; a plane test reads the 4-B record index of the leaf entry and 32 B of the
+# record, an inside test 32 B more.
 # ops: 6 (1/d) + 288 per expansion (8 x (24 slab + 12 octant)) + 25 per plane test
 #      + 140 per inside test (6 edge vectors, 3 dots, 3 sqrt, 3 IEEE div).
-K_RAY, K_OUT, K_GROUP, K_POP, K_LEAF, K_PLANE, K_INSIDE = 24, 8, 196, 8, 24, 32, 32
+K_RAY, K_OUT, K_GROUP, K_POP, K_LEAF, K_PLANE, K_INSIDE = 24, 8, 196, 8, 24, 36, 32
 KOPS_RAY, KOPS_GROUP, KOPS_PLANE, KOPS_INSIDE = 6, 288, 25, 140
 
 

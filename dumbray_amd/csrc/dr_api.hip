@@ -103,6 +103,8 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
   (void)hipGetDevice(&s->device);
   int rc = dr::dev_upload(s->groups, flat.groups.data(), flat.groups.size() * sizeof(float));
   if (rc == DR_OK) rc = dr::dev_upload(s->recs, flat.recs.data(), flat.recs.size() * sizeof(float));
+  if (rc == DR_OK) rc = dr::dev_upload(s->urecs, flat.urecs.data(), flat.urecs.size() * sizeof(float));
+  if (rc == DR_OK) rc = dr::dev_upload(s->lref, flat.lref.data(), flat.lref.size() * sizeof(int32_t));
   if (rc == DR_OK) {
     const std::vector<uint32_t> zeros(2 * dr::kWorkSlots, 0u);
     rc = dr::dev_upload(s->work, zeros.data(), zeros.size() * sizeof(uint32_t));
@@ -113,6 +115,8 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
   dr::DevTree &T = s->tree;
   T.groups = (const float4 *)s->groups.p;
   T.recs = (const float4 *)s->recs.p;
+  T.urecs = (const float4 *)s->urecs.p;
+  T.lref = (const int *)s->lref.p;
   T.rsx = flat.root.s.x; T.rsy = flat.root.s.y; T.rsz = flat.root.s.z;
   T.rex = flat.root.e.x; T.rey = flat.root.e.y; T.rez = flat.root.e.z;
   T.root_is_leaf = flat.root_is_leaf;
@@ -184,6 +188,8 @@ void dr_scene_destroy(dr_scene s) {
   if (!s) return;
   dr::dev_free(s->groups);
   dr::dev_free(s->recs);
+  dr::dev_free(s->urecs);
+  dr::dev_free(s->lref);
   dr::dev_free(s->shade);
   dr::dev_free(s->work);
   dr::dev_free(s->faces);

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
     const bool pend = r >= 0 && w.state == kWalkLeaf;
     const uint64_t pm = __ballot(pend);
     if (pm && (__popcll(pm) >= kLeafMin || __ballot(r >= 0 && w.state == kWalkTrace) == 0)) {
-      walk_leaf_wave(T, w, pend, clip != 0, best, ins, cc);
+      walk_leaf_wave<true>(T, w, pend, clip != 0, best, ins, cc);
       if (pend && w.state == kWalkDone) finish();
     }
   }

@@ -36,7 +36,7 @@ struct dr_octree_s {
 
 struct dr_scene_s {
   int device = 0;
-  dr::DevBuf groups, recs, shade;
+  dr::DevBuf groups, recs, urecs, lref, shade;
   dr::DevBuf faces;                        // per triangle: vertices a, b, c, then their normals (18 floats)
   std::vector<float> faces_host;
   dr::DevTree tree;
@@ -52,5 +52,5 @@ struct dr_scene_s {
   dr::DevBuf work;
   std::atomic<uint32_t> work_slot{0};
   int num_cu = 1;
-  int64_t bytes() const { return (int64_t)(groups.bytes + recs.bytes + shade.bytes + faces.bytes + work.bytes); }
+  int64_t bytes() const { return (int64_t)(groups.bytes + recs.bytes + urecs.bytes + lref.bytes + shade.bytes + faces.bytes + work.bytes); }
 };

@@ -51,7 +51,9 @@ constexpr int kRecWords = 16;
 
 struct OctreeFlat {
   std::vector<float> groups;               // ngroups * 64 words (ints stored bitwise)
-  std::vector<float> recs;                 // nrec * 16 words
+  std::vector<float> recs;                 // nrec * 16 words, one per leaf entry (leaf order)
+  std::vector<float> urecs;                // the same records once per distinct triangle
+  std::vector<int32_t> lref;               // leaf entry -> its record in urecs
   Box root;
   int root_is_leaf = 0, root_start = 0, root_count = 0;
   int ngroups = 0, nrec = 0, max_depth = 0;

@@ -513,12 +513,19 @@ int flatten_octree(const OctreeHost &t, OctreeFlat &out) {
     const Tri &tr = t.tris[id];
     return !(eq(tr.a, tr.b) || eq(tr.b, tr.c) || eq(tr.c, tr.a));    // degenerate never hits (Triangle.impl.h:292)
   };
+  std::vector<int32_t> urec_of(t.tris.size(), -1);
   auto leaf_records = [&](int node, int32_t &start, int32_t &count) {
     start = (int32_t)(out.recs.size() / kRecWords);
     count = 0;
     for (int k = t.list_off[node]; k < t.list_off[node + 1]; k++) {
-      if (!nondegenerate(t.list[k])) continue;
-      put_rec(out.recs, t.tris[t.list[k]], t.list[k]);
+      const int32_t id = t.list[k];
+      if (!nondegenerate(id)) continue;
+      put_rec(out.recs, t.tris[id], id);
+      if (urec_of[id] < 0) {
+        urec_of[id] = (int32_t)(out.urecs.size() / kRecWords);
+        put_rec(out.urecs, t.tris[id], id);
+      }
+      out.lref.push_back(urec_of[id]);
       count++;
     }
   };

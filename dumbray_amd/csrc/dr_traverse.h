@@ -18,7 +18,9 @@ namespace dr {
 
 struct DevTree {
   const float4 *__restrict__ groups;   // kGroupF4 float4 per group
-  const float4 *__restrict__ recs;     // 4 float4 per triangle record
+  const float4 *__restrict__ recs;     // 4 float4 per triangle record, one per leaf entry
+  const float4 *__restrict__ urecs;    // the same records once per distinct triangle
+  const int *__restrict__ lref;        // leaf entry -> record in urecs
   float rsx, rsy, rsz, rex, rey, rez;  // root box
   int root_is_leaf, root_start, root_count;
   float margin;                        // prune margin of the octant test (see the group layout)
@@ -223,6 +225,9 @@ __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *s
 //     first and by position second: the owner gets the first record with the smallest
 //     t, which is what the sequential strict '<' scan keeps (Octree.impl.cuh:595-606).
 //   * The owner then applies the leaf-box acceptance.
+// kIndexed: read the records through lref from urecs (a third of the footprint on
+// scenes whose triangles sit in several leaves; one more dependent load per record).
+template <bool kIndexed = false>
 __device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool pending, bool clip,
                                                unsigned long long *best, int *ins, CastCounters *cc) {
   const int lane = (int)__lane_id();
@@ -249,10 +254,11 @@ __device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool p
     const V3 o = v3(__shfl(w.ray.o.x, owner), __shfl(w.ray.o.y, owner), __shfl(w.ray.o.z, owner));
     const V3 d = v3(__shfl(w.ray.d.x, owner), __shfl(w.ray.d.y, owner), __shfl(w.ray.d.z, owner));
     const int pos = j - __shfl(start, owner);
-    const int k = __shfl(link, owner) + pos;
+    const int ke = __shfl(link, owner) + pos;
     const int ex = __shfl(w.ex, owner);
     if (j < total) {
-      const float4 *R = T.recs;
+      const int k = kIndexed ? T.lref[ke] : ke;
+      const float4 *R = kIndexed ? T.urecs : T.recs;
       const float4 r0 = R[4 * k + 0], r1 = R[4 * k + 1];
       const V3 a = v3(r0.x, r0.y, r0.z), n = v3(r0.w, r1.x, r1.y);
       const float dp = dot(a - o, n);
@@ -290,7 +296,7 @@ __device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool p
     else lb = child_box(T.groups, w.lg, w.lk);
     if (contains(lb, p)) {
       hit = true;
-      w.h.id = __float_as_int(T.recs[4 * k + 3].w);
+      w.h.id = kIndexed ? __float_as_int(T.urecs[4 * T.lref[k] + 3].w) : __float_as_int(T.recs[4 * k + 3].w);
       w.h.t = t;
       w.h.p = p;
     }
