@@ -46,11 +46,14 @@ constexpr int kGroupMask = 48, kGroupBase = 49, kGroupLink = 56, kGroupCount = 6
 //   a.x a.y a.z n.x | n.y n.z b.x b.y | b.z c.x c.y c.z | thr0 thr1 thr2 id
 // n = (b-a) x (c-a) and thr* = the ray-independent right-hand sides of the
 // containsVertex test; the plane test reads only the first 32 B.
-// Degenerate triangles (always a miss) get no record.
+// Degenerate triangles (always a miss) get no record.  Records exist in two layouts:
+// one per leaf entry in leaf order (read by the render kernels) and one per distinct
+// triangle with a leaf-entry -> record index (read by the closest-hit kernel, whose
+// random rays touch the whole scene: the compact layout fits the last-level cache).
 constexpr int kRecWords = 16;
 
 struct OctreeFlat {
-  std::vector<float> groups;               // ngroups * 64 words (ints stored bitwise)
+  std::vector<float> groups;               // ngroups * kGroupWords words (ints stored bitwise)
   std::vector<float> recs;                 // nrec * 16 words, one per leaf entry (leaf order)
   std::vector<float> urecs;                // the same records once per distinct triangle
   std::vector<int32_t> lref;               // leaf entry -> its record in urecs
