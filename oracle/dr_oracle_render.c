@@ -284,7 +284,8 @@ static void *render_worker(void *arg) {
 			if (J->mode == 0) pixel_color_f(C, px, py, J->out + 3 * k);
 			else if (J->mode == 1) pixel_color_d(C, px, py, J->out64 + 3 * k);
 			else if (J->mode == 3) pixel_color_ref(C, px, py, J->out + 3 * k);
-			else pixel_backward_d(C, px, py, J->dl + 3 * k, J->ga, J->ge);
+			else if (J->dl[3 * k] != 0.0f || J->dl[3 * k + 1] != 0.0f || J->dl[3 * k + 2] != 0.0f)
+				pixel_backward_d(C, px, py, J->dl + 3 * k, J->ga, J->ge);   /* dL/dI = 0 adds nothing */
 		}
 	return NULL;
 }
