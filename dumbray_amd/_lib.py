@@ -38,6 +38,8 @@ def _declare(L):
         "dr_scene_set_materials": (c_int, [vp, vp, c_int]),
         "dr_scene_set_normals": (c_int, [vp, vp, c_int]),
         "dr_scene_set_texcoords": (c_int, [vp, vp, c_int]),
+        "dr_shading_create": (c_int, [vp, vp, vp, vp, ctypes.POINTER(vp)]),
+        "dr_shading_destroy": (None, [vp]),
         "dr_render_blocks": (c_int, [vp, vp, vp, c_int, c_int, vp]),
         "dr_backward": (c_int, [vp, vp, vp, vp, vp, vp]),
         "dr_batch_create": (c_int, [vp, c_int, vp, vp]),

@@ -22,6 +22,94 @@ void dev_free(DevBuf &b) {
   b.p = nullptr;
   b.bytes = 0;
 }
+
+// Copies host data over a device buffer of the same size once the device is idle (the
+// buffer's address stays, so launches and batches holding it stay valid).
+static int dev_overwrite(DevBuf &b, const void *host, size_t bytes) {
+  if (bytes != b.bytes) return fail(DR_ERR_ARG, "internal: table size changed");
+  if (bytes == 0) return DR_OK;
+  DR_HIP_CHECK(hipDeviceSynchronize());
+  DR_HIP_CHECK(hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice));
+  return DR_OK;
+}
+
+// Per-triangle shading record words kFaceWords: the vertices (from the scene), the vertex
+// normals (given, or the flat unit Triangle::normal (c - a) x (b - a), Triangle.impl.h:233-235)
+// and the texture coordinates (given, or zeros).
+static void fill_faces(const dr_scene_s &s, const float *normals, const float *uv, std::vector<float> &F) {
+  F.assign((size_t)s.ntris * kFaceWords, 0.0f);
+  for (int i = 0; i < s.ntris; i++) {
+    float *f = &F[(size_t)i * kFaceWords];
+    memcpy(f, &s.faces_host[(size_t)i * kFaceWords], 9 * sizeof(float));
+    if (normals) {
+      memcpy(f + 9, normals + 9 * (size_t)i, 9 * sizeof(float));
+    } else {
+      const V3 a = v3(f[0], f[1], f[2]), b = v3(f[3], f[4], f[5]), c = v3(f[6], f[7], f[8]);
+      const V3 n = normalized(cross(c - a, b - a));
+      for (int k = 0; k < 3; k++) { f[9 + 3 * k] = n.x; f[10 + 3 * k] = n.y; f[11 + 3 * k] = n.z; }
+    }
+    if (uv)
+      for (int k = 0; k < 6; k++) f[18 + k] = uv[6 * (size_t)i + k];
+  }
+}
+
+// Material ids over the scene's per-triangle normals: returns the largest id, or -2 for a negative id.
+static int fill_shade(const dr_scene_s &s, const int32_t *mat_id, std::vector<float4> &out) {
+  out = s.shade_host;
+  int mx = -1;
+  for (int i = 0; i < s.ntris; i++) {
+    if (mat_id[i] < 0) return -2;
+    memcpy(&out[i].w, &mat_id[i], 4);
+    mx = mat_id[i] > mx ? mat_id[i] : mx;
+  }
+  return mx;
+}
+
+void scene_retain(dr_scene s) { s->refs.fetch_add(1); }
+void scene_release(dr_scene s) {
+  if (s->refs.fetch_sub(1) != 1) return;
+  dev_free(s->groups);
+  dev_free(s->recs);
+  dev_free(s->urecs);
+  dev_free(s->lref);
+  dev_free(s->shade);
+  dev_free(s->work);
+  dev_free(s->faces);
+  for (auto &e : s->scratch) dev_free(e.buf);
+  delete s;
+}
+void shading_retain(dr_shading h) { h->refs.fetch_add(1); }
+void shading_release(dr_shading h) {
+  if (h->refs.fetch_sub(1) != 1) return;
+  dev_free(h->shade);
+  dev_free(h->faces);
+  if (h->scene) scene_release(h->scene);
+  delete h;
+}
+
+int scene_scratch(dr_scene s, hipStream_t stream, size_t bytes, void **out) {
+  std::lock_guard<std::mutex> lock(s->scratch_mu);
+  dr_scene_s::StreamScratch *e = nullptr;
+  for (auto &x : s->scratch)
+    if (x.stream == stream) e = &x;
+  if (!e) {
+    s->scratch.push_back({stream, DevBuf{}});
+    e = &s->scratch.back();
+  }
+  if (e->buf.bytes < bytes) {
+    // earlier calls on this stream may still read the old buffer
+    DR_HIP_CHECK(hipStreamSynchronize(stream));
+    dev_free(e->buf);
+    const hipError_t err = hipMalloc(&e->buf.p, bytes);
+    if (err != hipSuccess) {
+      e->buf.p = nullptr;
+      return fail(DR_ERR_ALLOC, std::string("dr_backward: vertex records: ") + hipGetErrorString(err));
+    }
+    e->buf.bytes = bytes;
+  }
+  *out = e->buf.p;
+  return DR_OK;
+}
 }  // namespace dr
 
 extern "C" {
@@ -135,7 +223,12 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
     const float v[9] = {tr.a.x, tr.a.y, tr.a.z, tr.b.x, tr.b.y, tr.b.z, tr.c.x, tr.c.y, tr.c.z};
     memcpy(F, v, sizeof(v));
   }
-  rc = dr_scene_set_normals(s, nullptr, s->ntris);
+  // the scene's own tables: material id 0 (not yet usable: dr_scene_set_materials) and flat normals
+  std::vector<float> faces;
+  dr::fill_faces(*s, nullptr, nullptr, faces);
+  s->faces_host = faces;
+  rc = dr::dev_upload(s->shade, s->shade_host.data(), s->shade_host.size() * sizeof(float4));
+  if (rc == DR_OK) rc = dr::dev_upload(s->faces, faces.data(), faces.size() * sizeof(float));
   if (rc != DR_OK) { dr_scene_destroy(s); return rc; }
   *out = s;
   return DR_OK;
@@ -143,57 +236,74 @@ int dr_scene_create(dr_octree t, dr_scene *out) {
 
 int dr_scene_set_normals(dr_scene s, const float *normals, int ntris) {
   if (!s || ntris != s->ntris) return dr::fail(DR_ERR_ARG, "dr_scene_set_normals: one normal triple per triangle required");
-  for (int i = 0; i < ntris; i++) {
-    float *F = &s->faces_host[(size_t)i * dr::kFaceWords];
-    if (normals) {
-      memcpy(F + 9, normals + 9 * (size_t)i, 9 * sizeof(float));
-    } else {
-      // flat: the unit Triangle::normal (c - a) x (b - a) at every vertex
-      const dr::V3 a = dr::v3(F[0], F[1], F[2]), b = dr::v3(F[3], F[4], F[5]), c = dr::v3(F[6], F[7], F[8]);
-      const dr::V3 n = dr::normalized(dr::cross(c - a, b - a));
-      for (int k = 0; k < 3; k++) { F[9 + 3 * k] = n.x; F[10 + 3 * k] = n.y; F[11 + 3 * k] = n.z; }
-    }
-  }
-  dr::dev_free(s->faces);
-  return dr::dev_upload(s->faces, s->faces_host.data(), s->faces_host.size() * sizeof(float));
+  std::vector<float> F;
+  std::vector<float> uv(6 * (size_t)ntris);
+  for (int i = 0; i < ntris; i++)
+    memcpy(&uv[6 * (size_t)i], &s->faces_host[(size_t)i * dr::kFaceWords + 18], 6 * sizeof(float));
+  dr::fill_faces(*s, normals, uv.data(), F);
+  const int rc = dr::dev_overwrite(s->faces, F.data(), F.size() * sizeof(float));
+  if (rc == DR_OK) s->faces_host = std::move(F);
+  return rc;
 }
 
 int dr_scene_set_texcoords(dr_scene s, const float *uv, int ntris) {
   if (!s || ntris != s->ntris) return dr::fail(DR_ERR_ARG, "dr_scene_set_texcoords: one coordinate triple per triangle required");
-  for (int i = 0; i < ntris; i++) {
-    float *F = &s->faces_host[(size_t)i * dr::kFaceWords];
-    for (int k = 0; k < 6; k++) F[18 + k] = uv ? uv[6 * (size_t)i + k] : 0.0f;
-  }
-  dr::dev_free(s->faces);
-  return dr::dev_upload(s->faces, s->faces_host.data(), s->faces_host.size() * sizeof(float));
+  std::vector<float> F = s->faces_host;
+  for (int i = 0; i < ntris; i++)
+    for (int k = 0; k < 6; k++) F[(size_t)i * dr::kFaceWords + 18 + k] = uv ? uv[6 * (size_t)i + k] : 0.0f;
+  const int rc = dr::dev_overwrite(s->faces, F.data(), F.size() * sizeof(float));
+  if (rc == DR_OK) s->faces_host = std::move(F);
+  return rc;
 }
 
 int dr_scene_set_materials(dr_scene s, const int32_t *mat_id, int ntris) {
   if (!s || ntris != s->ntris || (ntris > 0 && !mat_id)) return dr::fail(DR_ERR_ARG, "dr_scene_set_materials: one id per triangle required");
-  int mx = -1;
-  for (int i = 0; i < ntris; i++) {
-    if (mat_id[i] < 0) return dr::fail(DR_ERR_ARG, "dr_scene_set_materials: negative material id");
-    memcpy(&s->shade_host[i].w, &mat_id[i], 4);
-    mx = mat_id[i] > mx ? mat_id[i] : mx;
-  }
-  s->max_mat = mx;
-  dr::dev_free(s->shade);
-  const int rc = dr::dev_upload(s->shade, s->shade_host.data(), sizeof(float4) * (size_t)ntris);
+  std::vector<float4> sh;
+  const int mx = dr::fill_shade(*s, mat_id, sh);
+  if (mx == -2) return dr::fail(DR_ERR_ARG, "dr_scene_set_materials: negative material id");
+  const int rc = dr::dev_overwrite(s->shade, sh.data(), sizeof(float4) * (size_t)ntris);
   if (rc != DR_OK) return rc;
+  s->shade_host = std::move(sh);
+  s->max_mat = mx;
   s->has_materials = true;
   return DR_OK;
 }
 
+int dr_shading_create(dr_scene s, const int32_t *mat_id, const float *normals, const float *uv, dr_shading *out) {
+  if (!s || !out || (s->ntris > 0 && !mat_id)) return dr::fail(DR_ERR_ARG, "dr_shading_create: bad arguments");
+  *out = nullptr;
+  dr_shading h = new (std::nothrow) dr_shading_s();
+  if (!h) return dr::fail(DR_ERR_ALLOC, "dr_shading_create: out of host memory");
+  std::vector<float4> sh;
+  std::vector<float> F;
+  int rc = DR_OK;
+  try {
+    h->max_mat = s->ntris > 0 ? dr::fill_shade(*s, mat_id, sh) : -1;
+    if (h->max_mat == -2) rc = dr::fail(DR_ERR_ARG, "dr_shading_create: negative material id");
+    if (rc == DR_OK) dr::fill_faces(*s, normals, uv, F);
+  } catch (const std::bad_alloc &) {
+    rc = dr::fail(DR_ERR_ALLOC, "dr_shading_create: out of host memory");
+  }
+  if (rc == DR_OK) rc = dr::dev_upload(h->shade, sh.data(), sh.size() * sizeof(float4));
+  if (rc == DR_OK) rc = dr::dev_upload(h->faces, F.data(), F.size() * sizeof(float));
+  if (rc != DR_OK) {
+    dr::dev_free(h->shade);
+    dr::dev_free(h->faces);
+    delete h;
+    return rc;
+  }
+  dr::scene_retain(s);
+  h->scene = s;
+  *out = h;
+  return DR_OK;
+}
+
+void dr_shading_destroy(dr_shading h) {
+  if (h) dr::shading_release(h);
+}
+
 void dr_scene_destroy(dr_scene s) {
-  if (!s) return;
-  dr::dev_free(s->groups);
-  dr::dev_free(s->recs);
-  dr::dev_free(s->urecs);
-  dr::dev_free(s->lref);
-  dr::dev_free(s->shade);
-  dr::dev_free(s->work);
-  dr::dev_free(s->faces);
-  delete s;
+  if (s) dr::scene_release(s);
 }
 
 int64_t dr_scene_device_bytes(dr_scene s) { return s ? s->bytes() : 0; }

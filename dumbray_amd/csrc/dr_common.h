@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/dumbray.h"
@@ -35,6 +36,9 @@ struct dr_octree_s {
 };
 
 struct dr_scene_s {
+  // references: the caller's (dr_scene_destroy drops it), one per dr_shading handle and per
+  // batch request; the scene's memory is freed with the last one
+  std::atomic<int> refs{1};
   int device = 0;
   dr::DevBuf groups, recs, urecs, lref, shade;
   dr::DevBuf faces;                        // per triangle: vertices a, b, c, then their normals (18 floats)
@@ -52,5 +56,30 @@ struct dr_scene_s {
   dr::DevBuf work;
   std::atomic<uint32_t> work_slot{0};
   int num_cu = 1;
+  // Adjoint vertex records, one buffer per stream that has run dr_backward on this scene
+  // (grown when a call needs more; calls on one stream are ordered, so they share it).
+  struct StreamScratch {
+    hipStream_t stream;
+    dr::DevBuf buf;
+  };
+  std::mutex scratch_mu;
+  std::vector<StreamScratch> scratch;
   int64_t bytes() const { return (int64_t)(groups.bytes + recs.bytes + urecs.bytes + lref.bytes + shade.bytes + faces.bytes + work.bytes); }
 };
+
+// One renderer's shading tables of a scene (dr_shading_create).
+struct dr_shading_s {
+  std::atomic<int> refs{1};
+  dr_scene scene = nullptr;               // holds a reference
+  int max_mat = -1;
+  dr::DevBuf shade, faces;                // same layouts as the scene's own tables
+};
+
+namespace dr {
+void scene_retain(dr_scene s);
+void scene_release(dr_scene s);
+void shading_retain(dr_shading h);
+void shading_release(dr_shading h);
+// The scene's vertex-record buffer for `stream`, at least `bytes` long.
+int scene_scratch(dr_scene s, hipStream_t stream, size_t bytes, void **out);
+}  // namespace dr

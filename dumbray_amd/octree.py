@@ -113,6 +113,29 @@ class DeviceScene:
         check(lib().dr_scene_create(octree._h, ctypes.byref(h)), "dr_scene_create")
         self._h = h
         self.ntris = int(octree.getData().shape[0])
+        self._shadings = {}
+
+    def shading(self, mat_id, normals=None, uv=None) -> "Shading":
+        """Shading tables of one renderer of this scene (dr_shading_create): material ids (ntris,),
+        vertex normals (ntris, 9) or None for flat normals, texture coordinates (ntris, 6) or None
+        for zeros.  Renderers with equal tables share one handle."""
+        import hashlib
+        import weakref
+
+        mid = np.ascontiguousarray(np.asarray(mat_id, np.int32).reshape(-1))
+        nrm = None if normals is None else np.ascontiguousarray(np.asarray(normals, np.float32).reshape(-1, 9))
+        tex = None if uv is None else np.ascontiguousarray(np.asarray(uv, np.float32).reshape(-1, 6))
+        if mid.shape[0] != self.ntris or (nrm is not None and nrm.shape[0] != self.ntris) or \
+                (tex is not None and tex.shape[0] != self.ntris):
+            raise DumbRayError("shading: one material id (and normal / texture coordinate row) per triangle required")
+        key = hashlib.sha1(b"".join([mid.tobytes(), b"n" if nrm is None else nrm.tobytes(),
+                                     b"u" if tex is None else tex.tobytes()])).hexdigest()
+        ref = self._shadings.get(key)
+        sh = ref() if ref is not None else None
+        if sh is None:
+            sh = Shading(self, mid, nrm, tex)
+            self._shadings[key] = weakref.ref(sh)
+        return sh
 
     @property
     def handle(self):
@@ -193,6 +216,32 @@ class DeviceScene:
         try:
             if self._h is not None:
                 lib().dr_scene_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class Shading:
+    """dr_shading handle: one renderer's material ids, vertex normals and texture coordinates of a
+    DeviceScene, in device memory of its own (dr_render_params.shading)."""
+
+    def __init__(self, scene: DeviceScene, mat_id, normals=None, uv=None):
+        h = ctypes.c_void_p()
+        ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        check(lib().dr_shading_create(scene.handle, ptr(mat_id), ptr(normals), ptr(uv), ctypes.byref(h)),
+              "dr_shading_create")
+        self._h = h
+        self.scene = scene
+        self.max_mat = int(mat_id.max()) if mat_id.size else -1
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                lib().dr_shading_destroy(self._h)
                 self._h = None
         except Exception:
             pass

@@ -38,7 +38,8 @@ class RenderParams(ctypes.Structure):
                 ("nmat", ctypes.c_int32), ("nlights", ctypes.c_int32), ("lens", ctypes.c_int32),
                 ("plugins", ctypes.c_int32), ("boxing", ctypes.c_int32), ("lens_p", ctypes.c_float * 8),
                 ("cam_rows", ctypes.c_float * 9), ("maps", ctypes.c_void_p), ("textures", ctypes.c_void_p),
-                ("texels", ctypes.c_void_p), ("nmaps", ctypes.c_int32), ("ntextures", ctypes.c_int32)]
+                ("texels", ctypes.c_void_p), ("nmaps", ctypes.c_int32), ("ntextures", ctypes.c_int32),
+                ("shading", ctypes.c_void_p)]
 
 
 BOXING = {"height": 0, "width": 1, "min": 2, "max": 3}   # DumbRenderer::BoxingMode (DumbRenderer.cuh:12-18)
@@ -145,11 +146,9 @@ class DumbRenderer:
         self.ignore_backfaces = bool(ignore_backfaces)
         self.seed = int(seed)
         self.camera = camera
-        mid = np.ascontiguousarray(np.asarray(mat_id, dtype=np.int32))
-        check(lib().dr_scene_set_materials(scene.handle, mid.ctypes.data_as(ctypes.c_void_p), int(mid.shape[0])),
-              "dr_scene_set_materials")
-        if normals is not None:
-            scene.set_normals(normals)
+        # this renderer's material ids, vertex normals and texture coordinates (the scene itself is
+        # shared with other renderers and stays untouched)
+        self.shading = scene.shading(mat_id, normals, uv)
         self.lens = LENSES[lens] if isinstance(lens, str) else int(lens)
         self.boxing = BOXING[boxing] if isinstance(boxing, str) else int(boxing)
         # SphericalSegmentLense parameters (plugins.spherical_segment_lens) and the camera's point-map
@@ -175,8 +174,6 @@ class DumbRenderer:
         self.maps_table = torch.from_numpy(maps.copy() if maps.size else np.zeros((1, 24), np.float32)).to(device)
         self.texture_table = torch.from_numpy(desc.copy() if desc.size else np.zeros((1, 4), np.int32)).to(device)
         self.texels = torch.from_numpy(texels.copy() if texels.size else np.zeros((1, 4), np.float32)).to(device)
-        if uv is not None:
-            scene.set_texcoords(uv)
 
     # ------------------------------------------------ BufferedRenderer / Renderer iteration interface
     # (Renderer.cu:136-156, BufferedRenderer.cu:7): iterate() renders iteration k + 1 into the
@@ -238,11 +235,10 @@ class DumbRenderer:
 
         with np.load(path, allow_pickle=False) as z:
             it, data, key = int(z["iteration"]), z["frame"], z["key"]
-        w, h = int(key[0]), int(key[1])
-        self.width, self.height = w, h
+        # the whole key, resolution included, is compared before anything of this renderer changes
         if not np.array_equal(key, self._progress_key()):
             raise DumbRayError("loadProgress: the file was written with other render settings")
-        frame = BlockBasedFrameBuffer(w, h, device)
+        frame = BlockBasedFrameBuffer(self.width, self.height, device)
         if tuple(frame.data.shape) != tuple(data.shape):
             raise DumbRayError("loadProgress: frame size differs from the settings")
         frame.data.copy_(torch.from_numpy(data))
@@ -281,6 +277,7 @@ class DumbRenderer:
         p.lens, p.plugins, p.boxing = self.lens, self.plugins, self.boxing
         p.maps, p.nmaps = self.maps_table.data_ptr(), self.nmaps
         p.textures, p.texels, p.ntextures = self.texture_table.data_ptr(), self.texels.data_ptr(), self.ntextures
+        p.shading = self.shading.handle.value
         return p
 
     def renderBlocksGPU(self, frame: BlockBasedFrameBuffer, start_block: int, end_block: int, iteration: int = 1,

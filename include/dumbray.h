@@ -32,6 +32,7 @@ enum {
 
 typedef struct dr_octree_s *dr_octree;   /* host-built octree (Octree<Renderable<BakedTriFace>>) */
 typedef struct dr_scene_s *dr_scene;     /* device-resident scene (geometry + shading tables) */
+typedef struct dr_shading_s *dr_shading; /* one renderer's shading tables of a scene (dr_shading_create) */
 
 /* Thread-local text of the last error (never NULL). */
 const char *dr_last_error(void);
@@ -209,10 +210,16 @@ typedef struct {
   const dr_texture *textures;     /* device table [ntextures], or NULL */
   const float *texels;            /* device RGBA fp32 texels of every texture */
   int32_t nmaps, ntextures;
+  dr_shading shading;             /* material ids, vertex normals and texture coordinates of this render
+                                     (dr_shading_create for the same scene), or NULL for the scene's own
+                                     (dr_scene_set_materials / _normals / _texcoords) */
 } dr_render_params;
 
 /* Attach per-triangle material ids (host int32[ntris], ntris = the octree's triangle
- * count; the scene keeps a device copy).  Needed before rendering. */
+ * count; the scene keeps a device copy).  Needed before rendering with the scene's own
+ * tables (dr_render_params.shading = NULL).  This and the two calls below wait for the
+ * device to finish its work, then overwrite the scene's tables in place: renders and
+ * batch replays issued afterwards read the new contents. */
 int dr_scene_set_materials(dr_scene s, const int32_t *mat_id, int ntris);
 
 /* Per-vertex normals of every triangle (BakedTriFace::norm, BakedTriMesh.h:8): host
@@ -225,6 +232,17 @@ int dr_scene_set_normals(dr_scene s, const float *normals, int ntris);
  * float[ntris * 6] (u, v of a, b, c), or NULL for zeros (the default).  Used by textured
  * materials of the reference shaders. */
 int dr_scene_set_texcoords(dr_scene s, const float *uv, int ntris);
+
+/* Shading tables of one renderer of scene `s`, in device memory owned by the handle:
+ * material ids (host int32[ntris], required), vertex normals (host float[ntris * 9] or NULL for
+ * flat normals) and texture coordinates (host float[ntris * 6] or NULL for zeros), with the
+ * layouts and defaults of dr_scene_set_materials / _normals / _texcoords.  A render passes the
+ * handle in dr_render_params.shading, so renderers of one scene may shade it differently at
+ * the same time (the reference's TriScene carries one material per Renderable element,
+ * Scene.cuh:68; DumbRenderer reads it at Material.impl.cuh:122-170).  The handle keeps its
+ * scene alive; a batch keeps the handles of its requests alive until dr_batch_destroy. */
+int dr_shading_create(dr_scene s, const int32_t *mat_id, const float *normals, const float *uv, dr_shading *out);
+void dr_shading_destroy(dr_shading h);
 
 /* DumbRenderer::renderBlocksGPU (DumbRenderer.cu:87-113, override of
  * BlockRenderer::renderBlocksGPU, BlockRenderer.cuh:43-44): renders framebuffer

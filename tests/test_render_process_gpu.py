@@ -221,3 +221,8 @@ def test_progress_checkpoint_resumes_bit_exactly(dev, tmp_path):
     other = ctx.renderer(W, H, tree=tree, samples_per_pixel=(1, 1))
     with pytest.raises(D.DumbRayError):
         other.loadProgress(path)
+    # another resolution is refused too, and the renderer keeps its own size
+    small = ctx.renderer(W // 2, H, tree=tree, samples_per_pixel=(2, 1))
+    with pytest.raises(D.DumbRayError):
+        small.loadProgress(path)
+    assert (small.width, small.height) == (W // 2, H) and small.getFrameBuffer() is None
