@@ -497,15 +497,46 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
 
 // ------------------------------------------------------------------ adjoint
 // Vertex records of each lane's current path ([vertex][word][lane] in the
-// workgroup's slice of a global scratch buffer): material id, reflection factor
-// A (3), direct sum d (3) with L_k = A (d + s L_{k+1}), and the sample weight s.
+// workgroup's slice of a scratch buffer): material id, reflection factor A (3),
+// direct sum d (3) with L_k = A (d + s L_{k+1}), the sample weight s, and dA/dalbedo
+// (1, or 1 - f5 for a Schlick metal, whose A = F0 + (1 - F0) f5).
 // The path is replayed from its Philox key; after it ends S_k = d_k + s_k L_{k+1}
-// is formed backwards and dL/dalbedo += w dL/dI beta_k S_k.  dL/demission is
+// is formed backwards and dL/dalbedo += w dL/dI beta_k dA_k S_k.  dL/demission is
 // added during the replay: w dL/dI beta_k A_k (cos_i/pi)(cos_l A/d^2).
 // Rays are step-wise walks as in render_block.  The grid is persistent: workgroup
-// g handles framebuffer blocks g, g + gridDim.x, ... and keeps its gradient sums
-// in LDS until the end, then adds them with one FP64 atomic per parameter.
-constexpr int kVtxWords = 8;
+// g handles framebuffer blocks g, g + gridDim.x, ...
+// Reductions: the lanes of a wave that add to the same parameter in the same path
+// update are summed across the wave first (one LDS atomic per distinct material or
+// light and channel); the workgroup's LDS sums of one framebuffer block then go to
+// the FP64 outputs with one global atomic per parameter, and restart from zero.
+constexpr int kVtxWords = 9;
+
+// Sum of v over the 64 lanes of the wave (butterfly; every lane gets the sum).
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Adds v[0..2] of every lane with `has` to acc[3 * key + c], summing lanes with equal
+// keys across the wave first.  Wave-uniform call.
+__device__ __forceinline__ void wave_add3(float *acc, bool has, int key, const float v[3]) {
+  const int lane = (int)__lane_id();
+  uint64_t act = __ballot(has);
+  while (act) {
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const int k = __shfl(key, leader);
+    const bool sel = has && key == k;
+    const float s0 = wave_sum(sel ? v[0] : 0.0f), s1 = wave_sum(sel ? v[1] : 0.0f), s2 = wave_sum(sel ? v[2] : 0.0f);
+    if (lane == leader) {
+      atomicAdd(&acc[3 * k + 0], s0);
+      atomicAdd(&acc[3 * k + 1], s1);
+      atomicAdd(&acc[3 * k + 2], s2);
+    }
+    has = has && !sel;
+    act = __ballot(has);
+  }
+}
 
 __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const float4 *__restrict__ shade,
                                                                  dr_render_params P, const float *__restrict__ dl,
@@ -538,34 +569,20 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
     Walk w;
     w.state = kWalkDone;
     int K = 0;                      // vertices recorded for the current path
+    bool fin = false;               // the current path has ended; its albedo terms are pending
     float wdl[3] = {0, 0, 0};       // spw * dL/dI (* lens filter) of the current pixel
     bool exhausted = false;
-    auto finish = [&]() {
-      float Ln[3] = {0, 0, 0};
-      for (int k = K - 1; k >= 0; k--)
-        for (int c = 0; c < 3; c++) {
-          const float S = V_(k, 4 + c) + V_(k, 7) * Ln[c];
-          Ln[c] = V_(k, 1 + c) * S;
-          V_(k, 4 + c) = S;
-        }
-      float beta[3] = {1, 1, 1};
-      for (int k = 0; k < K; k++) {
-        const int m = __float_as_int(V_(k, 0));
-        const int kind = P.materials[m].kind;
-        for (int c = 0; c < 3; c++) {
-          if (kind != DR_METAL) atomicAdd(&gacc[3 * m + c], wdl[c] * beta[c] * V_(k, 4 + c));
-          beta[c] = beta[c] * (V_(k, 1 + c) * V_(k, 7));
-        }
-      }
-      st.job = -1;
-    };
+    // the vertex's reflection factor and dA/dalbedo, then the next bounce (or the path's end)
     auto advance = [&]() {
-      float A[3];
-      reflect_factor(P, st, A);
-      for (int c = 0; c < 3; c++) V_(K - 1, 1 + c) = A[c];
+      const dr_material M = P.materials[st.mid];
+      float f5 = 0.0f;
+      const bool metal = st.kind == DR_METAL && st.has_sample;
+      if (metal) f5 = schlick5(st.nu, st.sdir);
+      for (int c = 0; c < 3; c++) V_(K - 1, 1 + c) = metal ? M.albedo[c] + (1.0f - M.albedo[c]) * f5 : M.albedo[c];
+      V_(K - 1, 8) = metal ? 1.0f - f5 : 1.0f;
       V_(K - 1, 7) = st.has_sample ? st.sweight : 0.0f;
-      if (!st.has_sample) { finish(); return; }
-      for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (A[c] * st.sweight);
+      if (!st.has_sample) { fin = true; return; }
+      for (int c = 0; c < 3; c++) st.beta[c] = st.beta[c] * (V_(K - 1, 1 + c) * st.sweight);
       st.o = st.p;
       st.d = st.sdir;
       st.restrict_tri = st.h;
@@ -611,10 +628,13 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
         walk_leaf_wave(T, w, pend, clip, best, nullptr, nullptr);
       const uint64_t dm = __ballot(st.job >= 0 && w.state == kWalkDone);
       if (dm && (__popcll(dm) >= kShadeMin || __ballot(st.job >= 0 && w.state != kWalkDone) == 0)) {
+        bool lit = false;
+        float ev[3] = {0.0f, 0.0f, 0.0f};
+        int lkey = 0;
         if (st.job >= 0 && w.state == kWalkDone) {
           bool photons = false;
           if (st.phase == 0) {
-            if (!w.hit) finish();
+            if (!w.hit) fin = true;
             else {
               setup_vertex(P, shade, w.h, w.ray.d, st);
               V_(K, 0) = __int_as_float(st.mid);
@@ -625,30 +645,67 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
             }
           } else {
             const V3 e = w.h.p - st.p;
-            const bool lit = w.hit && (w.h.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
+            lit = w.hit && (w.h.id == st.h || (e.x * e.x + e.y * e.y + e.z * e.z) <= (8.0f * kVecEps));
             if (lit) {
               const dr_material M = P.materials[st.mid];
               const float *E = P.lights[st.light].emission;
               for (int c = 0; c < 3; c++) {
                 V_(K - 1, 4 + c) += st.gl_k * E[c];
-                atomicAdd(&gacc[3 * P.nmat + 3 * st.light + c], wdl[c] * st.beta[c] * M.albedo[c] * st.gl_k);
+                ev[c] = wdl[c] * st.beta[c] * M.albedo[c] * st.gl_k;
               }
+              lkey = P.nmat + st.light;
             }
             photons = true;
           }
           if (photons && !next_photon(P, st)) advance();
-          if (st.job >= 0) walk_begin(T, w, st.o, st.d, st.restrict_tri, stk, kRB, nullptr);
+          if (!fin) walk_begin(T, w, st.o, st.d, st.restrict_tri, stk, kRB, nullptr);
+        }
+        // dL/demission of this update's lit photons
+        wave_add3(gacc, lit, lkey, ev);
+        // dL/dalbedo of the paths that ended: S_k backwards, then beta_k forwards
+        if (__ballot(fin)) {
+          if (fin) {
+            float Ln[3] = {0, 0, 0};
+            for (int k = K - 1; k >= 0; k--)
+              for (int c = 0; c < 3; c++) {
+                const float S = V_(k, 4 + c) + V_(k, 7) * Ln[c];
+                Ln[c] = V_(k, 1 + c) * S;
+                V_(k, 4 + c) = S;
+              }
+          }
+          float beta[3] = {1, 1, 1};
+          int kmax = fin ? K : 0;
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+          for (int k = 0; k < kmax; k++) {
+            const bool has = fin && k < K;
+            float av[3] = {0.0f, 0.0f, 0.0f};
+            int m = 0;
+            if (has) {
+              m = __float_as_int(V_(k, 0));
+              const float dA = V_(k, 8);
+              for (int c = 0; c < 3; c++) {
+                av[c] = wdl[c] * beta[c] * dA * V_(k, 4 + c);
+                beta[c] = beta[c] * (V_(k, 1 + c) * V_(k, 7));
+              }
+            }
+            wave_add3(gacc, has, m, av);
+          }
+          if (fin) { st.job = -1; fin = false; }
         }
       }
     }
     __syncthreads();
-  }
-  for (int t = threadIdx.x; t < ngrad; t += kRB) {
-    const float v = gacc[t];
-    if (v != 0.0f) {
-      if (t < 3 * P.nmat) atomicAdd(&g_albedo[t], (double)v);
-      else atomicAdd(&g_emission[t - 3 * P.nmat], (double)v);
+    // this block's sums into the FP64 gradients
+    for (int t = threadIdx.x; t < ngrad; t += kRB) {
+      const float v = gacc[t];
+      if (v != 0.0f) {
+        if (t < 3 * P.nmat) atomicAdd(&g_albedo[t], (double)v);
+        else atomicAdd(&g_emission[t - 3 * P.nmat], (double)v);
+      }
+      gacc[t] = 0.0f;
     }
+    __syncthreads();
   }
 }
 
@@ -692,12 +749,15 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
 
 static int check_params(dr_scene s, const dr_render_params *p) {
   if (!s || !p) return fail(DR_ERR_ARG, "render: null scene or parameters");
-  if (!s->has_materials) return fail(DR_ERR_ARG, "render: dr_scene_set_materials has not been called");
+  if (p->shading && p->shading->scene != s) return fail(DR_ERR_ARG, "render: shading tables of another scene");
+  if (!p->shading && !s->has_materials)
+    return fail(DR_ERR_ARG, "render: no material ids (dr_scene_set_materials or dr_render_params.shading)");
+  const int max_mat = p->shading ? p->shading->max_mat : s->max_mat;
   if (p->width < 1 || p->height < 1 || p->fsaa_x < 1 || p->fsaa_y < 1 || p->max_bounces < 0 || p->max_bounces > 64)
     return fail(DR_ERR_ARG, "render: bad image size, subpixel grid or max_bounces");
   if (p->nlights < 0 || p->nlights > kMaxLights || (p->nlights > 0 && !p->lights))
     return fail(DR_ERR_ARG, "render: 0 <= nlights <= 16 with a light table required");
-  if (!p->materials || p->nmat <= s->max_mat) return fail(DR_ERR_ARG, "render: material table smaller than the ids in use");
+  if (!p->materials || p->nmat <= max_mat) return fail(DR_ERR_ARG, "render: material table smaller than the ids in use");
   if (!(p->rr_survive > 0.0f && p->rr_survive <= 1.0f)) return fail(DR_ERR_ARG, "render: rr_survive must be in (0, 1]");
   if (s->max_depth > kMaxTreeDepth) return fail(DR_ERR_ARG, "render: octree deeper than the traversal stack");
   if (p->lens != DR_STOCHASTIC_LENS && p->lens != DR_PERSPECTIVE_LENS && p->lens != DR_SPHERICAL_SEGMENT_LENS)
@@ -721,6 +781,12 @@ static int render_group(int spp, int nblocks, int num_cu) {
 
 static int block_count(const dr_render_params *p) { return ((p->width + 7) / 8) * ((p->height + 15) / 16); }
 
+// The shading tables a render reads: its dr_shading handle's, or the scene's own.
+static Tables tables_of(dr_scene s, const dr_render_params *p) {
+  if (p->shading) return {(const float4 *)p->shading->shade.p, (const float *)p->shading->faces.p};
+  return {(const float4 *)s->shade.p, (const float *)s->faces.p};
+}
+
 }  // namespace dr
 
 extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
@@ -732,7 +798,7 @@ extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *fr
   if (start_block == end_block) return DR_OK;
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
   const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + (size_t)dr::kBlockPixels * dr::kRoundSamples * 3) * 4;
-  const dr::Tables tb = {(const float4 *)s->shade.p, (const float *)s->faces.p};
+  const dr::Tables tb = dr::tables_of(s, p);
   const int group = dr::render_group(p->fsaa_x * p->fsaa_y, end_block - start_block, s->num_cu);
   const int groups = (end_block - start_block + group - 1) / group;
   auto kernel = (p->plugins == DR_PLUGINS_REFERENCE)
@@ -744,34 +810,63 @@ extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *fr
   return DR_OK;
 }
 
+namespace dr {
+
+// Launch shape of the adjoint: LDS bytes (traversal stacks + gradient sums) and the size
+// of one workgroup's vertex records.
+static int backward_shape(dr_scene s, const dr_render_params *p, int &levels, int &maxv, size_t &lds, size_t &slice) {
+  levels = s->max_depth > 0 ? s->max_depth : 1;
+  maxv = p->max_bounces + 1;
+  lds = ((size_t)levels * kStackWordsPerLevel * kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
+  slice = (size_t)maxv * kVtxWords * kRB * sizeof(float);
+  if (lds > 160 * 1024) return fail(DR_ERR_ARG, "dr_backward: too many materials and lights for LDS");
+  if (lds > 64 * 1024)
+    DR_HIP_CHECK(hipFuncSetAttribute((const void *)k_backward, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return DR_OK;
+}
+
+// Workgroups of the persistent adjoint grid: one per resident slot, at most one per
+// framebuffer block, and at most as many as `vtx_bytes` of vertex records hold.
+static int backward_groups(dr_scene s, size_t lds, size_t slice, int nblocks, size_t vtx_bytes, int &groups) {
+  int per_cu = 0;
+  DR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_backward, kRB, lds));
+  const long long resident = (long long)s->num_cu * std::max(per_cu, 1);
+  groups = (int)std::min<long long>({(long long)nblocks, resident, (long long)(vtx_bytes / slice)});
+  return DR_OK;
+}
+
+static int check_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, const double *g_albedo,
+                          const double *g_emission) {
+  int rc = check_params(s, p);
+  if (rc != DR_OK) return rc;
+  if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return fail(DR_ERR_ARG, "dr_backward: null buffers");
+  if (p->plugins != DR_PLUGINS_BENCHMARK) return fail(DR_ERR_ARG, "dr_backward: only the benchmark plugin set has an adjoint");
+  return DR_OK;
+}
+
+static void launch_backward(dr_scene s, const dr_render_params *p, const float *dl, double *ga, double *ge, float *vtx,
+                            int groups, int nblocks, int levels, int maxv, size_t lds, hipStream_t stream) {
+  hipLaunchKernelGGL(k_backward, dim3(groups), dim3(kRB), lds, stream, s->tree, tables_of(s, p).shade, *p, dl, ga, ge,
+                     vtx, nblocks, levels, maxv);
+}
+
+}  // namespace dr
+
 extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo,
                            double *g_emission, void *stream) {
-  int rc = dr::check_params(s, p);
+  int rc = dr::check_backward(s, p, dl_dframe, g_albedo, g_emission);
   if (rc != DR_OK) return rc;
-  if (!dl_dframe || !g_albedo || (p->nlights > 0 && !g_emission)) return dr::fail(DR_ERR_ARG, "dr_backward: null buffers");
-  if (p->plugins != DR_PLUGINS_BENCHMARK) return dr::fail(DR_ERR_ARG, "dr_backward: only the benchmark plugin set has an adjoint");
-  const int levels = s->max_depth > 0 ? s->max_depth : 1;
-  const int maxv = p->max_bounces + 1;
-  const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + 3 * (size_t)(p->nmat + p->nlights)) * 4;
-  if (lds > 160 * 1024) return dr::fail(DR_ERR_ARG, "dr_backward: too many materials and lights for LDS");
-  if (lds > 64 * 1024) DR_HIP_CHECK(hipFuncSetAttribute((const void *)dr::k_backward,
-                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int levels, maxv, groups;
+  size_t lds, slice;
+  if ((rc = dr::backward_shape(s, p, levels, maxv, lds, slice)) != DR_OK) return rc;
   const int nblocks = dr::block_count(p);
-  if (nblocks == 0) return DR_OK;
-  int per_cu = 0;
-  DR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dr::k_backward, dr::kRB, lds));
-  const int groups = std::max(1, std::min(nblocks, s->num_cu * std::max(per_cu, 1)));
-  // vertex records: one slice of maxv * kVtxWords * kRB floats per workgroup, stream-ordered
-  const size_t vbytes = (size_t)groups * maxv * dr::kVtxWords * dr::kRB * sizeof(float);
+  if ((rc = dr::backward_groups(s, lds, slice, nblocks, SIZE_MAX, groups)) != DR_OK) return rc;
+  // vertex records: the scene's buffer for this stream, grown when a call needs more
   void *vtx = nullptr;
-  DR_HIP_CHECK(hipMallocAsync(&vtx, vbytes, (hipStream_t)stream));
-  hipLaunchKernelGGL(dr::k_backward, dim3(groups), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree,
-                     (const float4 *)s->shade.p, *p, dl_dframe, g_albedo, g_emission, (float *)vtx, nblocks, levels,
-                     maxv);
-  const hipError_t le = hipGetLastError();
-  const hipError_t fe = hipFreeAsync(vtx, (hipStream_t)stream);
-  DR_HIP_CHECK(le);
-  DR_HIP_CHECK(fe);
+  if ((rc = dr::scene_scratch(s, (hipStream_t)stream, (size_t)groups * slice, &vtx)) != DR_OK) return rc;
+  dr::launch_backward(s, p, dl_dframe, g_albedo, g_emission, (float *)vtx, groups, nblocks, levels, maxv, lds,
+                      (hipStream_t)stream);
+  DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
 }
 
@@ -782,6 +877,8 @@ struct dr_batch_s {
   dr_batch_tuning tuning{1, 2, 1};
   dr::DevBuf reqs, items[2], counter;
   std::vector<dr::DevRequest> host_reqs;
+  std::vector<dr_scene> scenes;      // references held until dr_batch_destroy
+  std::vector<dr_shading> shadings;
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -917,10 +1014,17 @@ extern "C" int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tun
     if (rc == DR_OK && !reqs[r].frame) rc = dr::fail(DR_ERR_ARG, "dr_batch_create: null frame");
     if (rc != DR_OK) break;
     dr::DevRequest d;
+    const dr::Tables tb = dr::tables_of(reqs[r].scene, &reqs[r].params);
     d.T = reqs[r].scene->tree;
-    d.shade = (const float4 *)reqs[r].scene->shade.p;
-    d.faces = (const float *)reqs[r].scene->faces.p;
+    d.shade = tb.shade;
+    d.faces = tb.faces;
     d.P = reqs[r].params;
+    dr::scene_retain(reqs[r].scene);
+    b->scenes.push_back(reqs[r].scene);
+    if (reqs[r].params.shading) {
+      dr::shading_retain(reqs[r].params.shading);
+      b->shadings.push_back(reqs[r].params.shading);
+    }
     d.frame = reqs[r].frame;
     b->host_reqs.push_back(d);
     b->levels = std::max(b->levels, reqs[r].scene->max_depth);
@@ -988,5 +1092,7 @@ extern "C" void dr_batch_destroy(dr_batch b) {
   dr::dev_free(b->items[0]);
   dr::dev_free(b->items[1]);
   dr::dev_free(b->counter);
+  for (dr_shading h : b->shadings) dr::shading_release(h);
+  for (dr_scene sc : b->scenes) dr::scene_release(sc);
   delete b;
 }

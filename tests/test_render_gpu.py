@@ -127,6 +127,41 @@ def test_backward_matches_fp64_oracle_and_fd(dev, oracle, lens):
             assert abs(fd - ga[m, c]) <= 1e-3 * abs(ga[m, c]) + 1e-4 * scale_a, (m, c, fd, ga[m, c])
 
 
+def test_backward_metal_albedo(dev, oracle):
+    """Schlick metal (A = F0 + (1 - F0) f5) gets dL/dF0 = sum of w dL/dI beta (1 - f5) S: the GPU
+    adjoint against the FP64 oracle adjoint and against central differences of GPU renders."""
+    import torch
+    sc = S.icosphere_scene(level=2, seed=3)
+    W, H = 48, 32
+    print(f"[case] adjoint icospheres (Lambert, mirror, metal) {W}x{H} spp 2x2 depth 4", flush=True)
+    g, o = _pair(oracle, sc, W, H, (2, 2), 4)
+    wimg = np.random.default_rng(4).uniform(-1, 1, (H, W, 3)).astype(np.float32)
+    dl = D.BlockBasedFrameBuffer(W, H).from_image(torch.from_numpy(wimg).cuda())
+    ga, ge = g.backward(dl)
+    torch.cuda.synchronize()
+    ga, ge = ga.cpu().numpy(), ge.cpu().numpy()
+    oa, oe = o.backward(wimg)
+    scale = np.abs(oa).max()
+    assert np.all(np.abs(ga - oa) <= 1e-4 * np.abs(oa) + 1e-6 * scale), (ga, oa)
+    assert np.all(np.abs(ge - oe) <= 1e-4 * np.abs(oe) + 1e-9), (ge, oe)
+    metal = [i for i, m in enumerate(sc.materials) if m["kind"] == "metal"][0]
+    assert np.abs(ga[metal]).min() > 1e-3 * scale
+
+    def loss():
+        _, img = _gpu_image(g)
+        return float(np.sum(img.astype(np.float64) * wimg))
+    base = g.albedo.clone()
+    for c in range(3):
+        up, dn = np.float32(float(base[metal, c]) + 1e-3), np.float32(float(base[metal, c]) - 1e-3)
+        g.albedo[metal, c] = float(up)
+        lp = loss()
+        g.albedo[metal, c] = float(dn)
+        lm = loss()
+        g.albedo.copy_(base)
+        fd = (lp - lm) / (float(up) - float(dn))
+        assert abs(fd - ga[metal, c]) <= 1e-3 * abs(ga[metal, c]), (c, fd, ga[metal, c])
+
+
 def test_autograd_function(dev):
     import torch
     from dumbray_amd.autograd import render
