@@ -41,6 +41,10 @@ K_RAY, K_OUT, K_GROUP, K_POP, K_LEAF, K_PLANE, K_INSIDE = 24, 8, 196, 8, 24, 36,
 KOPS_RAY, KOPS_GROUP, KOPS_PLANE, KOPS_INSIDE = 6, 288, 25, 140
 
 
+# host threads of the CPU legs: the cores this process may run on (the lease's share of the host)
+HOST_THREADS = len(os.sched_getaffinity(0))
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -98,7 +102,7 @@ def bench_path_trace(D, S, stream, cpu_seconds, with_cpu):
     if with_cpu:
         import oracle
         ot = oracle.Octree(sc.tris, 16, 12)
-        threads = min(os.cpu_count() or 1, 16)
+        threads = HOST_THREADS
         o = oracle.Renderer(ot, sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(), W, H, spp, depth,
                             rr_start=3, rr_survive=0.75)
         # a band of rows through the middle of the frame, grown until ~cpu_seconds/4
@@ -156,7 +160,7 @@ def bench_scene_file(D, stream, with_cpu, cpu_seconds):
         o = oracle.Renderer(oracle.Octree(sf.tris, 8, 24), sf.mat_id, sf.materials, sf.lights, sf.cam, W, H, sf.fsaa,
                             sf.max_bounces, ignore_backfaces=sf.ignore_backfaces, normals=sf.normals,
                             boxing=sf.boxing)
-        threads = min(os.cpu_count() or 1, 16)
+        threads = HOST_THREADS
         rows = 16
         while True:
             r0 = H // 2 - rows // 2
@@ -237,22 +241,16 @@ def bench_batch(D, S, stream):
     {1, 4, 16, 64}, depth from {2, 4, 8}; launch shape chosen by the autotuner.  Compared with
     rendering the same requests one dr_render_blocks call at a time."""
     import torch
-    rng = np.random.default_rng(2024)
-    scenes = []
-    for k in range(8):
-        sc = S.icosphere_scene(level=int(rng.integers(5, 8)), seed=100 + k)
-        scenes.append((sc, _device_scene(D, sc.tris)))
+    specs, qs = S.batch_workload()
+    scenes = [(sc, _device_scene(D, sc.tris)) for sc in specs]
     reqs = []
     total = 0
-    for i in range(64):
-        sc, ds = scenes[i % 8]
-        res = int([128, 256, 512, 1024][rng.integers(0, 4)])
-        f = int([1, 2, 4, 8][rng.integers(0, 4)])
-        depth = int([2, 4, 8][rng.integers(0, 3)])
-        r = D.DumbRenderer(ds, sc.mat_id, sc.materials, sc.lights, sc.camera, res, res, (f, f), depth, rr_start=3,
-                           rr_survive=0.75, seed=7000 + i)
-        reqs.append((r, D.BlockBasedFrameBuffer(res, res), 1, i + 1))
-        total += res * res * f * f
+    for q in qs:
+        sc, ds = scenes[q["scene"]]
+        r = D.DumbRenderer(ds, sc.mat_id, sc.materials, sc.lights, sc.camera, q["width"], q["height"], q["fsaa"],
+                           q["depth"], rr_start=3, rr_survive=0.75, seed=q["seed"])
+        reqs.append((r, D.BlockBasedFrameBuffer(q["width"], q["height"]), 1, q["request"]))
+        total += q["width"] * q["height"] * q["fsaa"][0] * q["fsaa"][1]
     t0 = time.perf_counter()
     batch = D.RenderBatch(reqs)
     tune_s = time.perf_counter() - t0
@@ -369,7 +367,7 @@ def main():
         oracle.build()
         log("oracle: building the reference octree on the host")
         ot = oracle.Octree(tris, 16, 12)
-        threads = min(os.cpu_count() or 1, 16)
+        threads = HOST_THREADS
         rng = np.random.default_rng(11)
         idx = np.sort(rng.choice(len(incoherent), 65536, replace=False))
         ores = ot.cast(incoherent[idx], counters=True, nthreads=threads)

@@ -363,6 +363,25 @@ def icosphere_scene(level: int = 7, seed: int = 3) -> Scene:
                  mats, lights, cam)
 
 
+def batch_workload(n_requests: int = 64, n_scenes: int = 8, seed: int = 2024):
+    """Batched uneven requests: n_scenes seeded scenes built like icosphere_scene at subdivision
+    levels 5-7 (61,442-983,042 tris each), and n_requests requests over them (request i uses scene
+    i % n_scenes) with resolution from {128^2, 256^2, 512^2, 1024^2}, samples per pixel from
+    {1, 4, 16, 64}, max depth from {2, 4, 8}, Russian roulette from depth 3 (survival 0.75).
+    Returns (scenes, requests); a request is a dict of scene, width, height, fsaa, depth, seed and
+    request (the Philox counter word)."""
+    rng = np.random.default_rng(seed)
+    scenes = [icosphere_scene(level=int(rng.integers(5, 8)), seed=100 + k) for k in range(n_scenes)]
+    reqs = []
+    for i in range(n_requests):
+        res = int([128, 256, 512, 1024][rng.integers(0, 4)])
+        f = int([1, 2, 4, 8][rng.integers(0, 4)])
+        depth = int([2, 4, 8][rng.integers(0, 3)])
+        reqs.append(dict(scene=i % n_scenes, width=res, height=res, fsaa=(f, f), depth=depth, seed=7000 + i,
+                         request=i + 1))
+    return scenes, reqs
+
+
 def random_soup(n: int, seed: int = 0, extent: float = 10.0, size: float = 1.0) -> np.ndarray:
     """Random triangle soup for parity tests (float32)."""
     rng = np.random.Generator(np.random.Philox(seed))

@@ -14,8 +14,57 @@ using namespace dumbray;
 template <typename T>
 static bool rd(FILE *f, T *p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
 
+// The failure path at the reference's call site: renderBlocksGPU returns false (the
+// reference then renders the blocks with renderBlocksCPU, DumbRenderer.cu:109-111); here the
+// library reports why, leaves the frame as it was and keeps working for the next call.
+static int failure_demo() {
+  try {
+    check(dr_device_open(0), "dr_device_open");
+    Octree<> tree(16, 12);
+    std::vector<Renderable<BakedTriFace>> faces(2);
+    faces[0].object.vert = Triangle{{-4, -1, -4}, {-4, -1, 4}, {4, -1, 4}};
+    faces[1].object.vert = Triangle{{-4, -1, -4}, {4, -1, 4}, {4, -1, -4}};
+    faces[0].materialId = 0;
+    faces[1].materialId = 1;                     // the renderer below has one material only
+    tree.push(faces);
+    tree.build();
+    dr_light l{};
+    l.kind = DR_AREA_LIGHT;
+    const float corner[3] = {-0.5f, 3.0f, -0.5f}, eu[3] = {1, 0, 0}, ev[3] = {0, 0, 1}, em[3] = {10, 10, 10};
+    for (int k = 0; k < 3; k++) { l.corner[k] = corner[k]; l.edge_u[k] = eu[k]; l.edge_v[k] = ev[k]; l.emission[k] = em[k]; }
+    const Camera c{{0, 2, -6}, {1, 0, 0}, {0, 0.9486833f, 0.3162278f}, {0, -0.3162278f, 0.9486833f}, 0.8660254f};
+    std::vector<dr_material> one = {Plugins::lambert({0.5f, 0.5f, 0.5f})};
+    DumbRenderer bad(tree, one, {l}, c, 2, 1, 1, false);
+    BlockBuffer frame(24, 20);
+    const size_t bytes = sizeof(float) * 4 * (size_t)frame.getBlockSize() * frame.getBlockCount();
+    hip_check(hipMemset(frame.deviceData(), 0x7f, bytes), "hipMemset");
+    if (bad.renderBlocksGPU(frame, 0, frame.getBlockCount(), 1)) throw Error("renderBlocksGPU accepted a short material table");
+    printf("renderBlocksGPU failed: %s\n", dr_last_error());
+    std::vector<unsigned char> raw(bytes);
+    hip_check(hipMemcpy(raw.data(), frame.deviceData(), bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (unsigned char b : raw)
+      if (b != 0x7f) throw Error("the failed call wrote the frame");
+    printf("frame untouched\n");
+    // the caller's fallback: here, a renderer with a complete material table
+    std::vector<dr_material> two = {one[0], Plugins::lambert({0.2f, 0.7f, 0.3f})};
+    DumbRenderer good(tree, two, {l}, c, 2, 1, 1, false);
+    if (!good.renderBlocksGPU(frame, 0, frame.getBlockCount(), 1)) throw Error(dr_last_error());
+    hip_check(hipDeviceSynchronize(), "sync");
+    const std::vector<float> img = frame.toHostImage();
+    float mx = 0.0f;
+    for (size_t i = 0; i < img.size(); i += 4) mx = img[i] > mx ? img[i] : mx;
+    if (!(mx > 0.0f)) throw Error("retry rendered nothing");
+    printf("retry rendered (max red %g)\n", mx);
+  } catch (const std::exception &e) {
+    fprintf(stderr, "host_api_demo --failure: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
-  if (argc != 3) { fprintf(stderr, "usage: %s scene.bin out.bin\n", argv[0]); return 2; }
+  if (argc == 2 && strcmp(argv[1], "--failure") == 0) return failure_demo();
+  if (argc != 3) { fprintf(stderr, "usage: %s scene.bin out.bin | --failure\n", argv[0]); return 2; }
   FILE *f = fopen(argv[1], "rb");
   if (!f) { perror("scene"); return 2; }
   int32_t hdr[10];
