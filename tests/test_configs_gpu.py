@@ -69,11 +69,12 @@ def test_path_trace_1080p_band_matches_oracle(dev, oracle):
     img = frame.to_image()[..., :3].cpu().numpy()
     o = oracle.Renderer(oracle.Octree(sc.tris, 16, 12), sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(),
                         W, H, spp, depth, rr_start=3, rr_survive=0.75)
-    for r0 in (520, 300):            # rows through the spheres and the ground, and above them
+    assert np.isfinite(img).all() and img.max() > 0
+    lit = np.where(img.max(axis=(1, 2)) > 0)[0]          # rows with geometry in view
+    for r0 in (int(lit[len(lit) // 2]) - 8, int(lit[(3 * len(lit)) // 4]) - 8):
         ref = o.render(rows=(r0, r0 + 16))
         assert ref.max() > 0
         _close(img[r0:r0 + 16], ref, f"rows {r0}..{r0 + 15}")
-    assert np.isfinite(img).all() and img.max() > 0
 
 
 def _batch_renderers():
@@ -163,7 +164,7 @@ def test_adjoint_512_matches_fd_and_fp64_window(dev, oracle):
     ga, ge = ga.cpu().numpy(), ge.cpu().numpy()
     eps = 1e-3
 
-    def fd(table, m, c):
+    def fd(table, m, c, eps):
         base = float(table[m, c])
         up, dn = np.float32(base + eps), np.float32(base - eps)
         table[m, c] = float(up)
@@ -175,11 +176,14 @@ def test_adjoint_512_matches_fd_and_fp64_window(dev, oracle):
     top = np.argsort(-np.abs(ga).max(axis=1))[:8]
     for m in top:
         for c in range(3):
-            v = fd(g.albedo, m, c)
+            v = fd(g.albedo, m, c, eps)
             assert abs(v - ga[m, c]) <= 1e-3 * abs(ga[m, c]), ("albedo", m, c, v, ga[m, c])
+    # The image is linear in emission, so a central difference has no truncation error and
+    # only the fp32 rounding of the two renders limits it; the step is 1e-3 relative to the
+    # emission (~1e-2) rather than absolute, which keeps that rounding 10x below the tolerance.
     for l in range(4):
         for c in range(3):
-            v = fd(g.emission, l, c)
+            v = fd(g.emission, l, c, eps * abs(float(g.emission[l, c])))
             assert abs(v - ge[l, c]) <= 1e-3 * abs(ge[l, c]), ("emission", l, c, v, ge[l, c])
     print(f"[case] FD: 8 albedo rows (|g| {np.abs(ga[top]).min():.3e}..{np.abs(ga[top]).max():.3e}) and 4 emitters "
           f"within 1e-3", flush=True)

