@@ -122,7 +122,7 @@ int dr_device_open(int device) {
   DR_HIP_CHECK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return dr::fail(DR_ERR_ARG, "dr_device_open: no such device");
   DR_HIP_CHECK(hipSetDevice(device));
-  return DR_OK;
+  return dr::noalloc_scratch_init(device, nullptr);
 }
 
 int dr_octree_build(const float *tris, int ntris, int split_threshold, int max_depth, int nthreads,

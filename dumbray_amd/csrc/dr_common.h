@@ -82,4 +82,6 @@ void shading_retain(dr_shading h);
 void shading_release(dr_shading h);
 // The scene's vertex-record buffer for `stream`, at least `bytes` long.
 int scene_scratch(dr_scene s, hipStream_t stream, size_t bytes, void **out);
+// Device address of dr_backward_noalloc's scratch array on `device` (resolved once per device).
+int noalloc_scratch_init(int device, float **out);
 }  // namespace dr

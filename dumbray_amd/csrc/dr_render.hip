@@ -852,6 +852,55 @@ static void launch_backward(dr_scene s, const dr_render_params *p, const float *
 
 }  // namespace dr
 
+namespace dr {
+
+// Vertex records of dr_backward_noalloc: a file-scope device array (kNoAllocBytes on every
+// device the library runs on, held until the process ends).  Host code never passes
+// `scratch` itself (its host-side name is not a device address): it passes the address
+// hipGetSymbolAddress gives for the current device, resolved once per device.
+constexpr size_t kNoAllocBytes = 40u << 20;
+static __device__ __align__(256) float scratch[kNoAllocBytes / sizeof(float)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_init(int device, float **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(DR_ERR_ARG, "dr_backward_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(DR_ERR_HIP, std::string("dr_backward_noalloc: scratch address: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (float *)ptr;
+  }
+  if (out) *out = g_scratch_ptr[device];
+  return DR_OK;
+}
+
+}  // namespace dr
+
+extern "C" int dr_backward_noalloc(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo,
+                                   double *g_emission, void *stream) {
+  int rc = dr::check_backward(s, p, dl_dframe, g_albedo, g_emission);
+  if (rc != DR_OK) return rc;
+  int levels, maxv, groups, device = 0;
+  size_t lds, slice;
+  if ((rc = dr::backward_shape(s, p, levels, maxv, lds, slice)) != DR_OK) return rc;
+  if (slice > dr::kNoAllocBytes)
+    return dr::fail(DR_ERR_ARG, "dr_backward_noalloc: one workgroup's vertex records exceed the 40 MiB scratch array");
+  DR_HIP_CHECK(hipGetDevice(&device));
+  float *vtx = nullptr;
+  if ((rc = dr::noalloc_scratch_init(device, &vtx)) != DR_OK) return rc;
+  const int nblocks = dr::block_count(p);
+  // as many persistent workgroups as the array holds vertex records for
+  if ((rc = dr::backward_groups(s, lds, slice, nblocks, dr::kNoAllocBytes, groups)) != DR_OK) return rc;
+  dr::launch_backward(s, p, dl_dframe, g_albedo, g_emission, vtx, groups, nblocks, levels, maxv, lds,
+                      (hipStream_t)stream);
+  DR_HIP_CHECK(hipGetLastError());
+  return DR_OK;
+}
+
 extern "C" int dr_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo,
                            double *g_emission, void *stream) {
   int rc = dr::check_backward(s, p, dl_dframe, g_albedo, g_emission);
