@@ -64,6 +64,83 @@ def read_traffic(path):
         return None
 
 
+# Counter passes this run launches (rocprofv3 --pmc, one child process each, on the profile-only
+# mode of this script: the timed incoherent casts and a 256 MiB calibration copy).  Counter
+# budget per pass: FETCH_SIZE = 3 TCC counters, WRITE_SIZE = 2, TCC_HIT/MISS 1 each, VALUBusy and
+# VALUUtilization = 2 SQ + 1 GRBM counters.
+PMC_PASSES = (("fetch", "FETCH_SIZE VALUBusy VALUUtilization"), ("write", "WRITE_SIZE TCC_HIT_sum TCC_MISS_sum"))
+CALIB_BYTES = 268435456
+CAST_KERNEL = "k_cast<false>"
+
+
+def _read_counters(outdir):
+    """{kernel name: {counter: [(dispatch id, value)]}} from rocprofv3 counter_collection.csv files."""
+    import csv
+    import glob
+    res = {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                name, ctr, val = row.get("Kernel_Name", ""), row.get("Counter_Name", ""), row.get("Counter_Value", "")
+                try:
+                    res.setdefault(name, {}).setdefault(ctr, []).append((int(row.get("Dispatch_Id", 0)), float(val)))
+                except ValueError:
+                    pass
+    return res
+
+
+def measure_counters(steps=3):
+    """Fabric bytes, L2 hit rate, VALU busy and VALU lane utilization of the cast kernel on the
+    incoherent batch, from rocprofv3 counter passes run by this invocation on this GPU; None
+    when rocprofv3 is missing or a pass fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    base = tempfile.mkdtemp(prefix="bench_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    per = {}
+    for tag, counters in PMC_PASSES:
+        out = os.path.join(base, tag)
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", *counters.split(), "--output-format", "csv", "-d", out,
+               "-o", tag, "--", sys.executable, os.path.abspath(__file__), "--profile", "--calib-copy", "--steps",
+               str(steps), "--warmup", "1"]
+        log(f"counter pass {tag}: {counters}")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        data = _read_counters(out)
+        if not data:
+            log(f"counter pass {tag} gave nothing (rc {r.returncode}): {r.stderr[-400:]}")
+            return None
+        for k, v in data.items():
+            per.setdefault(k, {}).update(v)
+    shutil.rmtree(base, ignore_errors=True)
+    cast = next((v for k, v in per.items() if CAST_KERNEL in k), None)
+    if not cast:
+        return None
+    # the calibration copy is the last copy kernel of the profile run (uploads of the workload
+    # also run as copy kernels, earlier)
+    copies = {}
+    for k, v in per.items():
+        if "copyBuffer" in k:
+            for ctr, xs in v.items():
+                copies.setdefault(ctr, []).extend(xs)
+    if "FETCH_SIZE" not in copies or "WRITE_SIZE" not in copies:
+        return None
+    last = lambda xs: max(xs)[1]
+    mean = lambda xs: float(np.mean([x for _, x in xs]))
+    f_fac = CALIB_BYTES / (last(copies["FETCH_SIZE"]) * 1024)
+    w_fac = CALIB_BYTES / (last(copies["WRITE_SIZE"]) * 1024)
+    fetch = mean(cast["FETCH_SIZE"]) * 1024 * f_fac
+    write = mean(cast["WRITE_SIZE"]) * 1024 * w_fac
+    hit, miss = mean(cast["TCC_HIT_sum"]), mean(cast["TCC_MISS_sum"])
+    return {"fetch_bytes": fetch, "write_bytes": write, "l2_hit_rate": hit / (hit + miss),
+            "valu_busy": mean(cast["VALUBusy"]) / 100.0, "valu_lane_utilization": mean(cast["VALUUtilization"]) / 100.0,
+            "calibration": {"copy_bytes": CALIB_BYTES, "fetch_factor": round(f_fac, 4), "write_factor": round(w_fac, 4)},
+            "dispatches": len(cast["FETCH_SIZE"])}
+
+
 def _event_ms(fn, k, stream):
     import torch
     fn()
@@ -84,6 +161,27 @@ def _device_scene(D, tris):
     return t.upload()
 
 
+def render_roofline(cnt, ms, pixels):
+    """Roofline of a render from the work its kernel counted (dr_render_stats): the traversal
+    lane-ops of every cast (as for the cast kernel, plus 6 ops of ray setup per cast) against
+    the VALU peak, and the bytes those casts read (groups, leaf boxes and links, 32-B halves of
+    triangle records; leaf-ordered records, so no record index) plus the 16-B framebuffer store
+    of every pixel, against HBM.  Shading arithmetic (BSDF sampling, light sampling, Philox) is
+    not counted, so the VALU fraction understates the kernel's instruction use."""
+    seg, ge, lv, pl, ins = (cnt[k] for k in ("segments", "group_expansions", "leaves", "plane_tests", "inside_tests"))
+    ops = KOPS_RAY * seg + KOPS_GROUP * ge + KOPS_PLANE * pl + KOPS_INSIDE * ins
+    pops = max(ge - seg, 0) + lv
+    byts = K_GROUP * ge + K_POP * pops + K_LEAF * lv + (K_PLANE - 4) * pl + K_INSIDE * ins + 16 * pixels
+    t = ms / 1e3
+    return {"bound": "valu", "achieved": round(ops / t / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "T lane-ops/s", "frac": round(ops / t / 1e12 / VALU_PEAK_TOPS, 4),
+            "hbm_logical": {"achieved": round(byts / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)},
+            "per_frame": {"casts": seg, "photon_casts": cnt["photons"], "group_expansions": ge, "leaves": lv,
+                          "plane_tests": pl, "inside_tests": ins, "paths": cnt["paths"], "traversal_ops": ops,
+                          "bytes": byts}}
+
+
 def bench_path_trace(D, S, stream, cpu_seconds, with_cpu):
     """1920x1080, 16 spp (4x4), max depth 8, Russian roulette from depth 3 on three displaced
     level-7 icospheres (983,040 tris) + ground quad, Lambert / mirror / Schlick metal, two area lights."""
@@ -99,6 +197,9 @@ def bench_path_trace(D, S, stream, cpu_seconds, with_cpu):
                        "Lambert/mirror/Schlick metal, 2 area lights",
            "ms_per_frame": round(ms, 3), "mpix_spp_s": round(W * H * 16 / ms / 1e3, 2)}
     log(f"path trace 1080p16: {ms:.2f} ms/frame, {out['mpix_spp_s']} Mpix*spp/s")
+    _, cnt = g.render_stats()
+    out["roofline"] = render_roofline(cnt, ms, W * H)
+    log(f"path trace counters: {cnt}; roofline {out['roofline']}")
     if with_cpu:
         import oracle
         ot = oracle.Octree(sc.tris, 16, 12)
@@ -173,6 +274,27 @@ def bench_scene_file(D, stream, with_cpu, cpu_seconds):
         out["cpu_oracle"] = {"value": round(W * rows * spp / spent / 1e6, 4), "unit": "Mpix*spp/s", "cores": threads,
                              "kind": "port", "sample": f"rows {r0}..{r0 + rows - 1} of the {W}x{H} frame through the "
                                                        f"oracle's reference estimator, {spent:.2f} s"}
+    return out
+
+
+def bench_rmse(D, S, stream):
+    """The 256x256 config-0 scene (depth 4) rendered on the GPU at 4 spp (its own setting) and at
+    16 spp, against the FP64 oracle image at 4096 spp committed in tests/golden (made by
+    tests/golden/make_fp64_reference.py with independent Philox keys)."""
+    import torch
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "spheres8_256_fp64_4096spp.npy")).astype(np.float64)
+    sc = S.spheres_scene(8, seed=0)
+    ds = _device_scene(D, sc.tris)
+    out = {"reference": "FP64 oracle estimator, 4096 spp, 256x256, depth 4", "mean_radiance": round(float(ref.mean()), 6)}
+    for spp in ((2, 2), (4, 4)):
+        g = D.DumbRenderer(ds, sc.mat_id, sc.materials, sc.lights, sc.camera, 256, 256, spp, 4, seed=11)
+        f = g.render()
+        torch.cuda.synchronize()
+        d = f.to_image()[..., :3].cpu().numpy().astype(np.float64) - ref
+        n = spp[0] * spp[1]
+        out[f"{n}spp"] = {"rmse": float(f"{np.sqrt(np.mean(d ** 2)):.5g}"), "mean_difference": float(f"{d.mean():.3g}"),
+                          "mean_difference_se": float(f"{d.std() / np.sqrt(d.size):.3g}")}
+    log(f"rmse vs FP64 4096 spp: {out}")
     return out
 
 
@@ -283,6 +405,7 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing and adjoint legs")
     ap.add_argument("--adam-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-requests leg")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter passes of the cast kernel")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -392,20 +515,54 @@ def main():
                          f"threads, median of {len(rates)} runs ({spent:.1f} s); octree build excluded"}
         log(f"cpu baseline: {cpu['value']} Mrays/s on {threads} threads")
 
-    algo_bytes = kernel_work["bytes"] * len(incoherent)
-    achieved = algo_bytes / (kB_ms / 1e3) / 1e9
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "cast_incoherent_hf2M_traffic.txt"))
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "dr::k_cast<false>", "kernel_ms": round(kB_ms, 4),
-                "algo_bytes_per_launch": int(algo_bytes)}
-    ops = kernel_work["ops"] * len(incoherent)
-    valu = {"achieved": round(ops / (kB_ms / 1e3) / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
-            "unit": "T lane-ops/s", "frac": round(ops / (kB_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)}
+    # ---------------------------------------------------------------- roofline of the cast kernel
+    # Work the kernel did (dr_cast_stats on the same rays) against the VALU peak, the resource
+    # that binds it; its fabric bytes, L2 hit rate, VALU busy and lane utilization come from the
+    # counter passes this run launches (or from the committed profile when they cannot run).
+    n = len(incoherent)
+    t = kB_ms / 1e3
+    ops = kernel_work["ops"] * n
+    logical = kernel_work["bytes"] * n
+    pmc = None if args.no_pmc else measure_counters()
+    if pmc is not None:
+        traffic, src = pmc["fetch_bytes"] + pmc["write_bytes"], "rocprofv3 --pmc passes launched by this bench run"
+    else:
+        traffic = read_traffic(os.path.join(ROOT, "profiles", "cast_incoherent_hf2M_traffic.txt"))
+        src = "profiles/cast_incoherent_hf2M_traffic.txt (committed; the counter passes did not run)"
+    roofline = {"bound": "valu", "achieved": round(ops / t / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "T lane-ops/s", "frac": round(ops / t / 1e12 / VALU_PEAK_TOPS, 4),
+                "traffic": None if traffic is None else int(traffic),
+                "kernel": "dr::k_cast<false>", "kernel_ms": round(kB_ms, 4), "ops_per_launch": int(ops),
+                "binding": "instruction issue: the useful lane-ops above are the kernel's box, plane and inside "
+                           "tests; the rest of the issue slots go to idle lanes and control",
+                "counters": None if pmc is None else {k: (round(v, 4) if isinstance(v, float) else v)
+                                                      for k, v in pmc.items()},
+                "counters_source": src,
+                "hbm": None if traffic is None else {
+                    "achieved": round(traffic / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(traffic / t / 1e9 / HBM_PEAK_GBS, 4),
+                    "note": "FETCH_SIZE + WRITE_SIZE scaled by a copy of known size; they count every L2 miss, "
+                            "including those the last-level (Infinity) cache serves, so this bounds HBM from above"},
+                "logical_bytes": {"per_ray": round(kernel_work["bytes"], 1), "achieved": round(logical / t / 1e9, 1),
+                                  "unit": "GB/s", "note": "bytes the kernel's loads name; most are L2 hits"}}
+    reference_equivalent = None
+    if per_ray is not None:
+        reference_equivalent = {"bytes_per_ray": round(per_ray["bytes"], 1), "ops_per_ray": round(per_ray["ops"], 1),
+                                "bytes_rate": round(per_ray["bytes"] * n / t / 1e9, 1), "bytes_unit": "GB/s",
+                                "ops_rate": round(per_ray["ops"] * n / t / 1e12, 3), "ops_unit": "T lane-ops/s",
+                                "note": "the reference algorithm's work (boxes, leaf refs and triangle tests it "
+                                        "runs on these rays) at this kernel's ray rate; the kernel prunes most of "
+                                        "it, so these rates can exceed the hardware peaks"}
+
+    # clipBackfaces = true (the reference's default and the renderer's setting), same batch
+    kC_ms = _event_ms(lambda: scene.cast(rB, clip_backfaces=True, stream=stream), args.steps, stream)
+    clip_true = {"kernel_ms": round(kC_ms, 4), "mrays_s": round(n / kC_ms / 1e3, 2), "kernel": "dr::k_cast<true>"}
+    log(f"batch B, clipBackfaces=true: {kC_ms:.3f} ms/launch ({clip_true['mrays_s']} Mrays/s)")
 
     render = None
     if not args.no_render:
         render = {"path_trace_1080p": bench_path_trace(D, S, stream, args.cpu_seconds, not args.no_cpu),
+                  "rmse_vs_fp64_4096spp": bench_rmse(D, S, stream),
                   "path_trace_256": bench_small(D, S, stream),
                   "adjoint_512": bench_adjoint(D, S, stream, args.adam_steps),
                   "batch_64_requests": None if args.no_batch else bench_batch(D, S, stream),
@@ -431,9 +588,11 @@ def main():
         "incoherent_kernel_mrays_s": round(len(incoherent) / kB_ms / 1e3, 2),
         "octree_build_s": round(build_s, 3),
         "roofline": roofline,
-        "valu": valu,
+        "clip_backfaces_true": clip_true,
         "kernel_work_per_ray": kernel_work,
         "reference_work_per_ray": per_ray,
+        "reference_equivalent": reference_equivalent,
+        "host": {"cores_used": HOST_THREADS, "nproc": os.cpu_count()},
         "oracle_sample_mismatches": parity,
         "cpu_baseline": cpu,
         "mpix_spp_s_1080p": render["path_trace_1080p"]["mpix_spp_s"] if render else None,

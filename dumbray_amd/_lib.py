@@ -41,6 +41,7 @@ def _declare(L):
         "dr_shading_create": (c_int, [vp, vp, vp, vp, ctypes.POINTER(vp)]),
         "dr_shading_destroy": (None, [vp]),
         "dr_render_blocks": (c_int, [vp, vp, vp, c_int, c_int, vp]),
+        "dr_render_stats": (c_int, [vp, vp, vp, c_int, c_int, vp, vp]),
         "dr_backward": (c_int, [vp, vp, vp, vp, vp, vp]),
         "dr_backward_noalloc": (c_int, [vp, vp, vp, vp, vp, vp]),
         "dr_batch_create": (c_int, [vp, c_int, vp, vp]),

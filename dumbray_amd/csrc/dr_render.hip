@@ -131,6 +131,7 @@ struct BenchPath {
   __device__ __forceinline__ V3 ray_d() const { return st.d; }
   __device__ __forceinline__ int ray_excl() const { return st.restrict_tri; }
   __device__ __forceinline__ const float *color() const { return st.L; }
+  __device__ __forceinline__ bool photon() const { return st.phase == 1; }
   // Next bounce after the last photon of a vertex; false when the path ends.
   __device__ __forceinline__ bool advance(const dr_render_params &P) {
     if (!st.has_sample) return false;
@@ -205,6 +206,7 @@ struct RefPath {
   __device__ __forceinline__ V3 ray_d() const { return d; }
   __device__ __forceinline__ int ray_excl() const { return excl; }
   __device__ __forceinline__ const float *color() const { return L; }
+  __device__ __forceinline__ bool photon() const { return phase == 1; }
 
   // The vertex's shading record (reference shaders; only the point for the others).
   __device__ __forceinline__ RefHit hit_record(const dr_render_params &P, const Tables &tb,
@@ -352,7 +354,35 @@ struct BlockLds {
   float *acc;           // kBlockPixels * 3 pixel sums
   int *next_job;
   unsigned long long *best;   // this wave's leaf-test slots (walk_leaf_wave)
+  int *ins;             // this wave's inside-test counts (measurement launches only)
 };
+
+// Work counters of a render (measurement launches only), summed over the launch.
+enum : int {
+  kRcSegments = 0,      // casts: camera and bounce rays and light photons
+  kRcPhotons,           //   of which light photons (shadow rays)
+  kRcExpand,            // group expansions (8 child box tests)
+  kRcLeaves,            // leaves tested
+  kRcPlane,             // triangle plane tests
+  kRcInside,            // triangle inside tests
+  kRcPaths,             // path jobs (pixel samples) started
+  kRcCount
+};
+struct RenderCounters {
+  CastCounters cast = {0, 0, 0, 0};
+  int segments = 0, photons = 0, paths = 0;
+};
+// Adds one lane's counters to the launch's totals (every lane of the wave calls it).
+__device__ __forceinline__ void flush_counters(const RenderCounters &c, unsigned long long *out) {
+  const int v[kRcCount] = {c.segments, c.photons, c.cast.expand, c.cast.leaves, c.cast.plane, c.cast.inside, c.paths};
+#pragma unroll
+  for (int k = 0; k < kRcCount; k++) {
+    unsigned long long x = (unsigned long long)v[k];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+    if (__lane_id() == 0) atomicAdd(&out[k], x);
+  }
+}
 
 // Framebuffer pixel `lp` of block `block`: the colour of this iteration, set (iteration 1)
 // or blended with weight 1/iteration (DumbRenderer.cu:64, BlockBasedFrameBuffer.impl.cuh:125-156).
@@ -378,10 +408,11 @@ __device__ __forceinline__ void store_pixel(const dr_render_params &P, float *__
 // one block has fewer paths than the workgroup has lanes.
 // refill_min: an idle lane takes a new job once at least this many lanes of its wave
 // are idle (or the whole wave is).
-template <class PathT>
+template <class PathT, bool kStats = false>
 __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb, const dr_render_params &P,
                                              float *__restrict__ frame, int block, int nb, const BlockLds &lds,
-                                             int refill_min) {
+                                             int refill_min, RenderCounters *rc = nullptr) {
+  CastCounters *cc = kStats ? &rc->cast : nullptr;
   const int spp = P.fsaa_x * P.fsaa_y;
   const int npix = kBlockPixels * nb;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
@@ -432,6 +463,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
               finish(zero);
             } else {
               walk_begin(T, w, ps.ray_o(), ps.ray_d(), ps.ray_excl(), lds.stk, kRB, nullptr);
+              if (kStats) { rc->paths++; rc->segments++; }
             }
           } else {
             exhausted = true;
@@ -443,18 +475,22 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
         continue;
       }
       // ---- one traversal step of every ray in flight
-      if (job >= 0 && w.state == kWalkTrace) walk_step(T, w, lds.stk, kRB, nullptr);
+      if (job >= 0 && w.state == kWalkTrace) walk_step(T, w, lds.stk, kRB, cc);
       // ---- leaf tests, batched
       const bool pend = job >= 0 && w.state == kWalkLeaf;
       const uint64_t pm = __ballot(pend);
       if (pm && (__popcll(pm) >= kLeafMin || __ballot(job >= 0 && w.state == kWalkTrace) == 0))
-        walk_leaf_wave(T, w, pend, clip, lds.best, nullptr, nullptr);
+        walk_leaf_wave(T, w, pend, clip, lds.best, kStats ? lds.ins : nullptr, cc);
       // ---- path updates of the finished casts, batched
       const uint64_t dm = __ballot(job >= 0 && w.state == kWalkDone);
       if (dm && (__popcll(dm) >= kShadeMin || __ballot(job >= 0 && w.state != kWalkDone) == 0)) {
         if (job >= 0 && w.state == kWalkDone) {
-          if (ps.shade(P, tb, w)) finish(ps.color());
-          else walk_begin(T, w, ps.ray_o(), ps.ray_d(), ps.ray_excl(), lds.stk, kRB, nullptr);
+          if (ps.shade(P, tb, w)) {
+            finish(ps.color());
+          } else {
+            walk_begin(T, w, ps.ray_o(), ps.ray_d(), ps.ray_excl(), lds.stk, kRB, nullptr);
+            if (kStats) { rc->segments++; rc->photons += ps.photon() ? 1 : 0; }
+          }
         }
       }
     }
@@ -481,18 +517,23 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
 
 // kGrouped: workgroups take runs of `group` framebuffer blocks (low sample counts);
 // otherwise one block each, with the run length fixed at 1 at compile time.
-template <class PathT, bool kGrouped>
+template <class PathT, bool kGrouped, bool kStats = false>
 __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables tb, dr_render_params P,
                                                                float *__restrict__ frame, int start_block,
-                                                               int end_block, int group, int levels) {
+                                                               int end_block, int group, int levels,
+                                                               unsigned long long *__restrict__ counts = nullptr) {
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];
   __shared__ int next_job;
   __shared__ unsigned long long best[kRB];
+  __shared__ int ins[kStats ? kRB : 1];
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
-                      best + (threadIdx.x & ~63u)};
+                      best + (threadIdx.x & ~63u), kStats ? ins + (threadIdx.x & ~63u) : nullptr};
   const int first = start_block + (int)blockIdx.x * group;
-  render_block<PathT>(T, tb, P, frame, first, kGrouped ? min(group, end_block - first) : 1, L, kRenderRefillMin);
+  RenderCounters rc;
+  render_block<PathT, kStats>(T, tb, P, frame, first, kGrouped ? min(group, end_block - first) : 1, L,
+                              kRenderRefillMin, &rc);
+  if (kStats) flush_counters(rc, counts);
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -733,7 +774,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
   __shared__ int item;
   __shared__ unsigned long long best[kRB];
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
-                      best + (threadIdx.x & ~63u)};
+                      best + (threadIdx.x & ~63u), nullptr};
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(counter, 1);
     __syncthreads();
@@ -789,25 +830,40 @@ static Tables tables_of(dr_scene s, const dr_render_params *p) {
 
 }  // namespace dr
 
-extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
-                                void *stream) {
-  int rc = dr::check_params(s, p);
+namespace dr {
+static int launch_render(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
+                         unsigned long long *counts, void *stream) {
+  int rc = check_params(s, p);
   if (rc != DR_OK) return rc;
-  if (!frame || start_block < 0 || end_block > dr::block_count(p) || start_block > end_block)
-    return dr::fail(DR_ERR_ARG, "dr_render_blocks: bad frame or block range");
+  if (!frame || start_block < 0 || end_block > block_count(p) || start_block > end_block)
+    return fail(DR_ERR_ARG, "dr_render_blocks: bad frame or block range");
   if (start_block == end_block) return DR_OK;
   const int levels = s->max_depth > 0 ? s->max_depth : 1;
-  const size_t lds = ((size_t)levels * dr::kStackWordsPerLevel * dr::kRB + (size_t)dr::kBlockPixels * dr::kRoundSamples * 3) * 4;
-  const dr::Tables tb = dr::tables_of(s, p);
-  const int group = dr::render_group(p->fsaa_x * p->fsaa_y, end_block - start_block, s->num_cu);
+  const size_t lds = ((size_t)levels * kStackWordsPerLevel * kRB + (size_t)kBlockPixels * kRoundSamples * 3) * 4;
+  const Tables tb = tables_of(s, p);
+  const int group = render_group(p->fsaa_x * p->fsaa_y, end_block - start_block, s->num_cu);
   const int groups = (end_block - start_block + group - 1) / group;
-  auto kernel = (p->plugins == DR_PLUGINS_REFERENCE)
-                    ? (group > 1 ? dr::k_render<dr::RefPath, true> : dr::k_render<dr::RefPath, false>)
-                    : (group > 1 ? dr::k_render<dr::BenchPath, true> : dr::k_render<dr::BenchPath, false>);
-  hipLaunchKernelGGL(kernel, dim3(groups), dim3(dr::kRB), lds, (hipStream_t)stream, s->tree, tb, *p, frame,
-                     start_block, end_block, group, levels);
+  const bool ref = p->plugins == DR_PLUGINS_REFERENCE;
+  auto kernel = counts ? (ref ? (group > 1 ? k_render<RefPath, true, true> : k_render<RefPath, false, true>)
+                              : (group > 1 ? k_render<BenchPath, true, true> : k_render<BenchPath, false, true>))
+                       : (ref ? (group > 1 ? k_render<RefPath, true> : k_render<RefPath, false>)
+                              : (group > 1 ? k_render<BenchPath, true> : k_render<BenchPath, false>));
+  hipLaunchKernelGGL(kernel, dim3(groups), dim3(kRB), lds, (hipStream_t)stream, s->tree, tb, *p, frame, start_block,
+                     end_block, group, levels, counts);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
+}
+}  // namespace dr
+
+extern "C" int dr_render_blocks(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
+                                void *stream) {
+  return dr::launch_render(s, p, frame, start_block, end_block, nullptr, stream);
+}
+
+extern "C" int dr_render_stats(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
+                               uint64_t *counts, void *stream) {
+  if (!counts) return dr::fail(DR_ERR_ARG, "dr_render_stats: null counters");
+  return dr::launch_render(s, p, frame, start_block, end_block, (unsigned long long *)counts, stream);
 }
 
 namespace dr {

@@ -291,6 +291,24 @@ class DumbRenderer:
         check(lib().dr_render_blocks(self.scene.handle, ctypes.byref(p), _vp(frame.data.data_ptr()), int(start_block),
                                      int(end_block), _vp(s.cuda_stream)), "dr_render_blocks")
 
+    RENDER_COUNTERS = ("segments", "photons", "group_expansions", "leaves", "plane_tests", "inside_tests", "paths")
+
+    def render_stats(self, frame: BlockBasedFrameBuffer | None = None, iteration: int = 1, stream=None):
+        """dr_render_stats: renders like render() and returns (frame, counters of the work the
+        kernel did: casts, photon casts, group expansions, leaves, plane and inside tests, paths)."""
+        import torch
+
+        if frame is None:
+            frame = BlockBasedFrameBuffer(self.width, self.height, self.material_table.device)
+        counts = torch.zeros(len(self.RENDER_COUNTERS), dtype=torch.int64, device=frame.data.device)
+        p = self.params(iteration)
+        s = stream if stream is not None else torch.cuda.current_stream(frame.data.device)
+        check(lib().dr_render_stats(self.scene.handle, ctypes.byref(p), _vp(frame.data.data_ptr()), 0,
+                                    frame.getBlockCount(), _vp(counts.data_ptr()), _vp(s.cuda_stream)),
+              "dr_render_stats")
+        s.synchronize()
+        return frame, dict(zip(self.RENDER_COUNTERS, (int(v) for v in counts.cpu())))
+
     def render(self, frame: BlockBasedFrameBuffer | None = None, iteration: int = 1, stream=None):
         if frame is None:
             frame = BlockBasedFrameBuffer(self.width, self.height, self.material_table.device)

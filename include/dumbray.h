@@ -384,6 +384,14 @@ int dr_png_read(const char *path, uint8_t *rgba, int width, int height);
 int dr_cast_stats(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude, int32_t *out_id,
                   float *out_t, int32_t *out_counts, void *stream);
 
+/* dr_render_blocks plus work counters of the render (measurement only; no counterpart in the
+ * reference, which prints timings only).  counts: device uint64[7], added to (the caller zeroes
+ * it): casts (camera, bounce and photon rays), of which light photons, group expansions (8
+ * child box tests, 196 B read each), leaves tested, triangle plane tests and inside tests (32 B
+ * of a record each), and path jobs started.  Same frame as dr_render_blocks. */
+int dr_render_stats(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
+                    uint64_t *counts, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -213,3 +213,17 @@ def test_adjoint_skips_nothing_outside_a_window(oracle):
             fe += e
     assert np.allclose(ga, fa, rtol=1e-12, atol=1e-15) and np.allclose(ge, fe, rtol=1e-12, atol=1e-15)
     assert np.abs(ga).max() > 0
+
+
+def test_fp64_4096spp_fixture_is_the_oracle_render(oracle):
+    """tests/golden/spheres8_256_fp64_4096spp.npy is what its script renders (a band, recomputed)."""
+    import importlib.util
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("make_fp64_reference", os.path.join(here, "make_fp64_reference.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    fix = np.load(os.path.join(here, "spheres8_256_fp64_4096spp.npy"))
+    assert fix.shape == (256, 256, 3) and fix.dtype == np.float32
+    band = m.render(rows=(150, 153)).astype(np.float32)
+    assert band.max() > 0 and np.array_equal(band, fix[150:153])
