@@ -16,17 +16,6 @@ constexpr int kWaveChunk = 128;       // rays a wave takes from the global count
 constexpr int kRefillMin = 16;        // idle lanes that trigger a refill
 constexpr int kLeafMin = 16;          // pending leaves that trigger the (wave-shared) leaf tests
 
-__device__ __forceinline__ void write_result(int r, bool hit, const Hit &h, int *__restrict__ out_id,
-                                             float *__restrict__ out_t, float *__restrict__ out_p) {
-  out_id[r] = hit ? h.id : -1;
-  out_t[r] = hit ? h.t : FLT_MAX;
-  if (out_p) {
-    out_p[3 * (size_t)r + 0] = hit ? h.p.x : 0.0f;
-    out_p[3 * (size_t)r + 1] = hit ? h.p.y : 0.0f;
-    out_p[3 * (size_t)r + 2] = hit ? h.p.z : 0.0f;
-  }
-}
-
 // One bounded step per loop iteration.  A lane that is traversing pops one child:
 // an interior child is expanded and pushed, a leaf child is held as pending.
 // Pending leaves are tested together once at least kLeafMin lanes hold one (or
@@ -38,22 +27,40 @@ __device__ __forceinline__ void write_result(int r, bool hit, const Hit &h, int 
 // from the global counter work[0]; lanes that need a ray take the next ones of the
 // range, and the wave takes a new range when it runs out.  The last wave to exit
 // (counted in work[1]) sets both counters back to zero for the next launch.
+//
+// Results are staged per range in LDS (kSlots ranges per wave) and written out as
+// whole lines once no ray of the range is still being traced: rays end at scattered
+// times, and 4-B stores issued as they end would leave L2 as partial sectors.  When a
+// wave needs a slot for a new range while both hold unfinished ranges, the older one
+// is written out as far as it is done, and its remaining rays store their results
+// directly.
+constexpr int kSlots = 2;
+constexpr int kPendingId = -2;        // staged entry of a ray still being traced
+
 template <bool kStats>
-__global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__restrict__ rays, int n, int clip,
+__global__ __launch_bounds__(kCastBlock, 4) void k_cast(DevTree T, const float *__restrict__ rays, int n, int clip,
                                                       const int *__restrict__ exclude, int *__restrict__ out_id,
                                                       float *__restrict__ out_t, float *__restrict__ out_p,
                                                       int *__restrict__ stats, unsigned *__restrict__ work) {
   extern __shared__ uint32_t stk_base[];
   __shared__ unsigned long long best_all[kCastBlock];   // per lane: best (t, record) key of its leaf
   __shared__ int ins_all[kStats ? kCastBlock : 1];      // per lane: inside tests of its leaf (measurement)
+  __shared__ int res_id_all[kWavesPerBlock][kSlots][kWaveChunk];
+  __shared__ float res_t_all[kWavesPerBlock][kSlots][kWaveChunk];
   uint32_t *stk = stk_base + threadIdx.x;
   unsigned long long *best = best_all + (threadIdx.x & ~63u);
   int *ins = kStats ? ins_all + (threadIdx.x & ~63u) : nullptr;
+  int *res_id = &res_id_all[threadIdx.x >> 6][0][0];
+  float *res_t = &res_t_all[threadIdx.x >> 6][0][0];
   constexpr int S = kCastBlock;
 
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   unsigned wcur = 0, wend = 0;          // the wave's current range of rays (wave-uniform)
+  // staging slots (wave-uniform): slot k holds the results of range [base_k, base_k + len_k)
+  // while len_k > 0; `cur` is the slot of the current range
+  unsigned base0 = 0, base1 = 0;
+  int len0 = 0, len1 = 0, cur = 0;
   int r = -1;
   bool exhausted = false;
   Walk w;
@@ -61,7 +68,24 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
   CastCounters cnt_local = {0, 0, 0, 0};
   CastCounters *cc = kStats ? &cnt_local : nullptr;
   auto finish = [&]() {
-    write_result(r, w.hit, w.h, out_id, out_t, out_p);
+    const int id = w.hit ? w.h.id : -1;
+    const float t = w.hit ? w.h.t : FLT_MAX;
+    const unsigned u = (unsigned)r;
+    if (len0 > 0 && u - base0 < (unsigned)len0) {
+      res_id[u - base0] = id;
+      res_t[u - base0] = t;
+    } else if (len1 > 0 && u - base1 < (unsigned)len1) {
+      res_id[kWaveChunk + (u - base1)] = id;
+      res_t[kWaveChunk + (u - base1)] = t;
+    } else {
+      out_id[r] = id;
+      out_t[r] = t;
+    }
+    if (out_p) {
+      out_p[3 * (size_t)r + 0] = w.hit ? w.h.p.x : 0.0f;
+      out_p[3 * (size_t)r + 1] = w.hit ? w.h.p.y : 0.0f;
+      out_p[3 * (size_t)r + 2] = w.hit ? w.h.p.z : 0.0f;
+    }
     if (kStats) {
       stats[4 * (size_t)r + 0] = cnt_local.expand;
       stats[4 * (size_t)r + 1] = cnt_local.leaves;
@@ -71,8 +95,29 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
     }
     r = -1;
   };
+  // writes the finished entries of slot k (range base, len) as whole lines (wave-uniform)
+  auto flush = [&](int k, unsigned base, int len) {
+    __builtin_amdgcn_wave_barrier();
+    for (int j = lane; j < len; j += 64) {
+      const int id = res_id[k * kWaveChunk + j];
+      if (id != kPendingId) {
+        out_id[base + j] = id;
+        out_t[base + j] = res_t[k * kWaveChunk + j];
+      }
+    }
+  };
+  // a range whose rays have all been handed out and finished leaves its slot
+  auto retire = [&](int k, unsigned base, int &len) {
+    if (len > 0 && (k != cur || wcur == wend) &&
+        __ballot(r >= 0 && (unsigned)r - base < (unsigned)len) == 0) {
+      flush(k, base, len);
+      len = 0;
+    }
+  };
 
   for (;;) {
+    retire(0, base0, len0);
+    retire(1, base1, len1);
     // ---- refill idle lanes
     const bool want = (r < 0) && !exhausted;
     const uint64_t wmask = __ballot(want);
@@ -88,12 +133,24 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
         const int leader = __ffsll((unsigned long long)wmask) - 1;
         unsigned nb = 0;
         if (lane == leader) nb = atomicAdd(work, (unsigned)kWaveChunk);
-        nb = __shfl(nb, leader);
+        nb = (unsigned)__builtin_amdgcn_readlane((int)nb, leader);   // wave-uniform: kept in scalar registers
         const unsigned top = min(nb + (unsigned)kWaveChunk, (unsigned)n);
         const unsigned len = nb < top ? top - nb : 0u;
         slot = idx < rem ? (int)(wcur + idx) : (idx - rem < len ? (int)(nb + idx - rem) : -1);
         wcur = nb + min(need - rem, len);
         wend = nb + len;
+        if (len > 0) {
+          // the new range takes the slot the current range does not hold; an unfinished
+          // range still there is written out as far as it is done
+          const int k = (cur == 0 ? len0 : len1) > 0 ? 1 - cur : cur;
+          if (k == 0 && len0 > 0) flush(0, base0, len0);
+          if (k == 1 && len1 > 0) flush(1, base1, len1);
+          __builtin_amdgcn_wave_barrier();
+          for (int j = lane; j < kWaveChunk; j += 64) res_id[k * kWaveChunk + j] = kPendingId;
+          if (k == 0) { base0 = nb; len0 = (int)len; }
+          else { base1 = nb; len1 = (int)len; }
+          cur = k;
+        }
       }
       if (want) {
         if (slot >= 0) {
@@ -125,6 +182,9 @@ __global__ __launch_bounds__(kCastBlock) void k_cast(DevTree T, const float *__r
       if (pend && w.state == kWalkDone) finish();
     }
   }
+  // every ray of the wave has ended: the ranges still staged go out
+  if (len0 > 0) flush(0, base0, len0);
+  if (len1 > 0) flush(1, base1, len1);
   // ---- the last wave out resets the counters
   if (lane == 0) {
     __threadfence();

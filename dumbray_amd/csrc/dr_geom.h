@@ -72,6 +72,21 @@ DR_HD bool slab_hit(float sx, float sy, float sz, float ex, float ey, float ez, 
   return r != FLT_MAX;
 }
 
+// slab_hit for a ray whose 1/d components are all finite.  Then no (s - o) * inv product is a
+// NaN, min and max are exact (no rounding) and so associative, and the native min/max can
+// differ from std::min/max only in the sign of a zero result, which neither comparison below
+// sees: the decision is slab_hit's, bit for bit, with the IEEE min/max instructions.
+DR_HD bool slab_hit_finite(float sx, float sy, float sz, float ex, float ey, float ez, V3 o, V3 inv) {
+  const float x0 = (sx - o.x) * inv.x, x1 = (ex - o.x) * inv.x;
+  const float y0 = (sy - o.y) * inv.y, y1 = (ey - o.y) * inv.y;
+  const float z0 = (sz - o.z) * inv.z, z1 = (ez - o.z) * inv.z;
+  const float mn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
+                                   __builtin_fminf(z0, z1));
+  const float mx = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
+                                   __builtin_fmaxf(z0, z1));
+  return !(mn > mx + kVecEps) && mn != FLT_MAX;
+}
+
 // Per-triangle constants of Triangle::containsVertex that do not depend on the ray
 // (Triangle.impl.h:299-301 right-hand sides) and the plane normal (:335).
 struct TriPre { V3 n; float thr0, thr1, thr2; bool degenerate; };

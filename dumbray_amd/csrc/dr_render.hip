@@ -123,6 +123,7 @@ struct Tables {
 
 // Path updates of the benchmark plugins (DR_PLUGINS_BENCHMARK).
 struct BenchPath {
+  static constexpr bool kFastSlab = true;
   Path st;
   __device__ __forceinline__ bool start(const dr_render_params &P, int block, int lp, int s, uint32_t samp0) {
     return start_path(P, block, lp, s, samp0, st);
@@ -177,6 +178,9 @@ struct BenchPath {
 // sample are computed again where they are used; both are functions of the vertex and
 // of counter-based random numbers, so the order of evaluation does not change them.
 struct RefPath {
+  // one form of the child test: its registers are short, and photons of the reference's
+  // directional lights are often axis-parallel, which needs the NaN-handling form anyway
+  static constexpr bool kFastSlab = false;
   uint32_t pix, samp;
   int layer, phase, light, lk, excl, trans;   // lk: photons of the current light still to send
   V3 o, d;                          // the next ray to cast
@@ -475,7 +479,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
         continue;
       }
       // ---- one traversal step of every ray in flight
-      if (job >= 0 && w.state == kWalkTrace) walk_step(T, w, lds.stk, kRB, cc);
+      if (job >= 0 && w.state == kWalkTrace) walk_step<PathT::kFastSlab>(T, w, lds.stk, kRB, cc);
       // ---- leaf tests, batched
       const bool pend = job >= 0 && w.state == kWalkLeaf;
       const uint64_t pm = __ballot(pend);

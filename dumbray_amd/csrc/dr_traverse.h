@@ -102,6 +102,7 @@ constexpr int kStackWordsPerLevel = 2;
 // 8 child tests of group g -> stack entry.  A child is kept if it has a non-empty
 // leaf below it, its box widened by the prune margin meets the forward octant of
 // the ray (the thresholds of set_prune), and the reference slab test on its box passes.
+template <bool kFinite>
 __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g, const TRay &r, uint32_t &lo,
                                             uint32_t &hi) {
   const float4 *q = G + kGroupF4 * (size_t)g;
@@ -117,7 +118,9 @@ __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g,
       // forward octant of o: coordinate k of o + d*t (t >= 0) is >= o.k if d.k >= 0, <= o.k if d.k <= 0
       const bool behind = (a_ex[j] < r.hi.x) || (a_sx[j] > r.lo.x) || (a_ey[j] < r.hi.y) || (a_sy[j] > r.lo.y) ||
                           (a_ez[j] < r.hi.z) || (a_sz[j] > r.lo.z);
-      const bool hit = !behind && slab_hit(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv);
+      const bool hit =
+          !behind && (kFinite ? slab_hit_finite(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv)
+                              : slab_hit(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv));
       m |= hit ? (1u << (4 * h + j)) : 0u;
     }
   }
@@ -174,6 +177,9 @@ __device__ __forceinline__ void walk_begin(const DevTree &T, Walk &w, V3 o, V3 d
 // Pops one child: expands an interior child, or holds a leaf child as pending;
 // an exhausted stack ends the query with a miss.  A walk's first step (sp == -1 after
 // walk_begin) expands the root's group.
+// kFastSlab: expand with the NaN-free slab test whenever no expanding lane of the wave has an
+// infinite 1/d (kernels with registers to spare for the second copy of the child test).
+template <bool kFastSlab = true>
 __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *stk, int stride, CastCounters *cc) {
   int child = 0;                      // group to expand; 0 (the root's) on a walk's first step
   if (w.sp >= 0) {
@@ -202,7 +208,11 @@ __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *s
     child = (int)(hi >> 8) + __builtin_popcount(im & ((1u << k) - 1u));
   }
   uint32_t lo2, hi2;
-  group_entry(T.groups, child, w.ray, lo2, hi2);
+  // the NaN-free form of the slab test when no expanding lane of the wave has an infinite 1/d
+  const bool finite = __builtin_isfinite(w.ray.inv.x) && __builtin_isfinite(w.ray.inv.y) &&
+                      __builtin_isfinite(w.ray.inv.z);
+  if (kFastSlab && __ballot(!finite) == 0) group_entry<true>(T.groups, child, w.ray, lo2, hi2);
+  else group_entry<false>(T.groups, child, w.ray, lo2, hi2);
   ++w.sp;
   stk[2 * w.sp * stride] = lo2;
   stk[(2 * w.sp + 1) * stride] = hi2;
