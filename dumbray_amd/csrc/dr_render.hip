@@ -556,27 +556,59 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
 // the FP64 outputs with one global atomic per parameter, and restart from zero.
 constexpr int kVtxWords = 9;
 
-// Sum of v over the 64 lanes of the wave (butterfly; every lane gets the sum).
+// Sum of v over the 64 lanes of the wave, returned to every lane: an inclusive scan within
+// each row of 16 lanes with DPP row shifts (1, 2, 4, 8; a lane without a source adds 0)
+// leaves the row totals in lanes 15, 31, 47 and 63, which are read out and added.
+template <int kCtrl>
+__device__ __forceinline__ float dpp_or_zero(float v) {   // the DPP source's bits, 0 where it has none
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  v += dpp_or_zero<0x111>(v);   // row_shr:1
+  v += dpp_or_zero<0x112>(v);   // row_shr:2
+  v += dpp_or_zero<0x114>(v);   // row_shr:4
+  v += dpp_or_zero<0x118>(v);   // row_shr:8
+  return (lane_value(v, 15) + lane_value(v, 31)) + (lane_value(v, 47) + lane_value(v, 63));
 }
 
-// Adds v[0..2] of every lane with `has` to acc[3 * key + c], summing lanes with equal
-// keys across the wave first.  Wave-uniform call.
+// Lanes that must share a parameter before their terms are summed across the wave: a sum
+// costs six cross-lane steps per channel, a lane's own LDS atomic one contended add.
+// (1, 8, 24 and 65 measured 22.8 / 22.5 / 22.2 / 21.8 ms for the 512x512 16-spp adjoint of
+// 64 materials and 4 lights)
+constexpr int kWaveSumMin = 16;
+
+// Adds v[0..2] of every lane with `has` to acc[3 * key + c], one key at a time: the lanes
+// sharing a key are summed across the wave first (one atomic per channel) when there are at
+// least kWaveSumMin of them, and add their own terms otherwise.  Wave-uniform call.
 __device__ __forceinline__ void wave_add3(float *acc, bool has, int key, const float v[3]) {
   const int lane = (int)__lane_id();
   uint64_t act = __ballot(has);
+  if (__popcll(act) < kWaveSumMin) {          // too few terms for any sum across the wave to pay
+    if (has) {
+      atomicAdd(&acc[3 * key + 0], v[0]);
+      atomicAdd(&acc[3 * key + 1], v[1]);
+      atomicAdd(&acc[3 * key + 2], v[2]);
+    }
+    return;
+  }
   while (act) {
     const int leader = __ffsll((unsigned long long)act) - 1;
-    const int k = __shfl(key, leader);
+    const int k = __builtin_amdgcn_readlane(key, leader);
     const bool sel = has && key == k;
-    const float s0 = wave_sum(sel ? v[0] : 0.0f), s1 = wave_sum(sel ? v[1] : 0.0f), s2 = wave_sum(sel ? v[2] : 0.0f);
-    if (lane == leader) {
-      atomicAdd(&acc[3 * k + 0], s0);
-      atomicAdd(&acc[3 * k + 1], s1);
-      atomicAdd(&acc[3 * k + 2], s2);
+    if (__popcll(__ballot(sel)) >= kWaveSumMin) {
+      const float s0 = wave_sum(sel ? v[0] : 0.0f), s1 = wave_sum(sel ? v[1] : 0.0f), s2 = wave_sum(sel ? v[2] : 0.0f);
+      if (lane == leader) {
+        atomicAdd(&acc[3 * k + 0], s0);
+        atomicAdd(&acc[3 * k + 1], s1);
+        atomicAdd(&acc[3 * k + 2], s2);
+      }
+    } else if (sel) {
+      atomicAdd(&acc[3 * k + 0], v[0]);
+      atomicAdd(&acc[3 * k + 1], v[1]);
+      atomicAdd(&acc[3 * k + 2], v[2]);
     }
     has = has && !sel;
     act = __ballot(has);

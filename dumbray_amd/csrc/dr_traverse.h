@@ -75,13 +75,41 @@ __device__ __forceinline__ void set_prune(TRay &r, float M) {
             r.d.z <= 0 ? above(r.o.z + M) : inf);
 }
 
-// Triangle::containsVertex minus the degenerate test (degenerate triangles have no record)
+// One edge of Triangle::containsVertex (Triangle.impl.h:299-301): fl(fl(D / fl(sqrt(S))) + tol) >= thr
+// with D = dot(edge, p - vertex), S = dot(p - vertex, p - vertex), correctly rounded division
+// and square root.
+__device__ __forceinline__ bool edge_exact(float D, float S, float thr) {
+  return D / __builtin_sqrtf(S) + kInsideTol >= thr;
+}
+// The same decision from the hardware reciprocal square root where it is certain: 1 = pass,
+// 0 = fail, 2 = too close to call.  q = D * rsq(S) is within about 2 ulp of the correctly
+// rounded quotient (rsq within 1 ulp, one product rounding, the quotient's two roundings), and
+// the sum with tol adds at most one ulp more.  An edge is decided only when q + tol lies more
+// than kEdgeMargin * (|q| + tol), about 34 ulp, from thr; S outside [1e-30, 1e30] (where
+// denormals or overflow could enter) is left to the exact form.
+constexpr float kEdgeMargin = 4e-6f;
+__device__ __forceinline__ int edge_fast(float D, float S, float thr) {
+  if (!(S >= 1e-30f && S <= 1e30f)) return 2;
+  const float q = D * __builtin_amdgcn_rsqf(S);
+  const float r = q + kInsideTol;
+  const float m = kEdgeMargin * (__builtin_fabsf(q) + kInsideTol);
+  return (r - m >= thr) ? 1 : ((r + m < thr) ? 0 : 2);
+}
+
+// Triangle::containsVertex minus the degenerate test (degenerate triangles have no record).
+// The three edge conditions are ANDed; each is decided by edge_fast where it is certain and by
+// the exact form otherwise, so the result is the reference's bit for bit.
 __device__ __forceinline__ bool inside(V3 p, V3 a, V3 b, V3 c, float thr0, float thr1, float thr2) {
   if (eq(p, a) || eq(p, b) || eq(p, c)) return true;
   const V3 ab = b - a, bc = c - b, ca = a - c;
   const V3 ax = p - a, bx = p - b, cx = p - c;
-  return (dot(ab, ax) / mag(ax) + kInsideTol >= thr0) && (dot(bc, bx) / mag(bx) + kInsideTol >= thr1) &&
-         (dot(ca, cx) / mag(cx) + kInsideTol >= thr2);
+  const float D0 = dot(ab, ax), S0 = dot(ax, ax), D1 = dot(bc, bx), S1 = dot(bx, bx), D2 = dot(ca, cx),
+              S2 = dot(cx, cx);
+  const int e0 = edge_fast(D0, S0, thr0), e1 = edge_fast(D1, S1, thr1), e2 = edge_fast(D2, S2, thr2);
+  if (e0 == 0 || e1 == 0 || e2 == 0) return false;
+  if ((e0 & e1 & e2) == 1) return true;
+  return (e0 == 1 || edge_exact(D0, S0, thr0)) && (e1 == 1 || edge_exact(D1, S1, thr1)) &&
+         (e2 == 1 || edge_exact(D2, S2, thr2));
 }
 
 // Box of child k of group g (the leaf-acceptance box).
@@ -253,14 +281,21 @@ __device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool p
   if (pending) __hip_atomic_store(&best[lane], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   if (cc && pending) __hip_atomic_store(&ins[lane], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   const int link = pending ? w.llink : 0;
+  // The owner of record j is the last lane whose first record number is <= j (the numbers
+  // never decrease from lane to lane), i.e. the count of lanes 1..63 whose number is <= j.
+  // Level 1 counts among lanes 8, 16, ..., 56, whose numbers sit in scalar registers for the
+  // whole batch; level 2 among the 7 lanes after the multiple of 8 found, fetched together.
+  int coarse[7];
+#pragma unroll
+  for (int m = 0; m < 7; m++) coarse[m] = __builtin_amdgcn_readlane(start, 8 * (m + 1));
   for (int base = 0; base < total; base += 64) {
     const int j = base + lane;
-    int owner = 0;
+    int b = 0;
 #pragma unroll
-    for (int step = 32; step >= 1; step >>= 1) {
-      const int s = __shfl(start, owner + step);
-      if (s <= j) owner += step;
-    }
+    for (int m = 0; m < 7; m++) b += coarse[m] <= j ? 8 : 0;
+    int owner = b;
+#pragma unroll
+    for (int i = 1; i < 8; i++) owner += __shfl(start, b + i) <= j ? 1 : 0;
     const V3 o = v3(__shfl(w.ray.o.x, owner), __shfl(w.ray.o.y, owner), __shfl(w.ray.o.z, owner));
     const V3 d = v3(__shfl(w.ray.d.x, owner), __shfl(w.ray.d.y, owner), __shfl(w.ray.d.z, owner));
     const int pos = j - __shfl(start, owner);
@@ -269,14 +304,15 @@ __device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool p
     if (j < total) {
       const int k = kIndexed ? T.lref[ke] : ke;
       const float4 *R = kIndexed ? T.urecs : T.recs;
-      const float4 r0 = R[4 * k + 0], r1 = R[4 * k + 1];
+      // the whole 64-B record in one round trip: most records pass the plane test and need
+      // the second half too
+      const float4 r0 = R[4 * k + 0], r1 = R[4 * k + 1], r2 = R[4 * k + 2], r3 = R[4 * k + 3];
       const V3 a = v3(r0.x, r0.y, r0.z), n = v3(r0.w, r1.x, r1.y);
       const float dp = dot(a - o, n);
       const float dd = dot(d, n);
       const float t = dp / dd;
       if (!(clip && (dp < -kVecEps)) && !(dp * dd <= 0) && (t < FLT_MAX)) {
         if (cc) __hip_atomic_fetch_add(&ins[owner], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-        const float4 r2 = R[4 * k + 2], r3 = R[4 * k + 3];
         if (__float_as_int(r3.w) != ex) {
           const V3 p = o + d * t;
           const V3 b = v3(r1.z, r1.w, r2.x), c = v3(r2.y, r2.z, r2.w);
