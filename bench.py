@@ -35,10 +35,11 @@ OPS_RAY, OPS_BOX, OPS_TRI = 6, 24, 155
 # expansion reads 196 B (8 child boxes + the non-empty mask) and pops read 8 B (link, count); a leaf
 # reads its 24-B box; a plane test reads the 4-B record index of the leaf entry and 32 B of the
 # record, an inside test 32 B more.
-# ops: 6 (1/d) + 288 per expansion (8 x (24 slab + 12 octant)) + 25 per plane test
-#      + 140 per inside test (6 edge vectors, 3 dots, 3 sqrt, 3 IEEE div).
+# ops: 6 (1/d) + 12 per octant test (the children of an expanded group with a non-empty leaf below)
+#      + 24 per slab test (those not behind the origin) + 25 per plane test
+#      + 140 per inside test (6 edge vectors, 3 dots, 3 square roots, 3 divisions).
 K_RAY, K_OUT, K_GROUP, K_POP, K_LEAF, K_PLANE, K_INSIDE = 24, 8, 196, 8, 24, 36, 32
-KOPS_RAY, KOPS_GROUP, KOPS_PLANE, KOPS_INSIDE = 6, 288, 25, 140
+KOPS_RAY, KOPS_OCTANT, KOPS_SLAB, KOPS_PLANE, KOPS_INSIDE = 6, 12, 24, 25, 140
 
 
 # host threads of the CPU legs: the cores this process may run on (the lease's share of the host)
@@ -169,7 +170,8 @@ def render_roofline(cnt, ms, pixels):
     of every pixel, against HBM.  Shading arithmetic (BSDF sampling, light sampling, Philox) is
     not counted, so the VALU fraction understates the kernel's instruction use."""
     seg, ge, lv, pl, ins = (cnt[k] for k in ("segments", "group_expansions", "leaves", "plane_tests", "inside_tests"))
-    ops = KOPS_RAY * seg + KOPS_GROUP * ge + KOPS_PLANE * pl + KOPS_INSIDE * ins
+    ops = (KOPS_RAY * seg + KOPS_OCTANT * cnt["live_children"] + KOPS_SLAB * cnt["slab_tests"] + KOPS_PLANE * pl
+           + KOPS_INSIDE * ins)
     pops = max(ge - seg, 0) + lv
     byts = K_GROUP * ge + K_POP * pops + K_LEAF * lv + (K_PLANE - 4) * pl + K_INSIDE * ins + 16 * pixels
     t = ms / 1e3
@@ -178,7 +180,8 @@ def render_roofline(cnt, ms, pixels):
             "hbm_logical": {"achieved": round(byts / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)},
             "per_frame": {"casts": seg, "photon_casts": cnt["photons"], "group_expansions": ge, "leaves": lv,
-                          "plane_tests": pl, "inside_tests": ins, "paths": cnt["paths"], "traversal_ops": ops,
+                          "plane_tests": pl, "inside_tests": ins, "live_children": cnt["live_children"],
+                          "slab_tests": cnt["slab_tests"], "paths": cnt["paths"], "traversal_ops": ops,
                           "bytes": byts}}
 
 
@@ -470,11 +473,12 @@ def main():
     torch.cuda.synchronize()
     cnt = st["counts"].double().mean(dim=0).cpu().numpy()
     assert torch.equal(st["id"], outB["id"]), "dr_cast_stats disagrees with dr_cast"
-    ge, lv, pl, ins = (float(x) for x in cnt)
+    ge, lv, pl, ins, live, slab = (float(x) for x in cnt)
     kernel_work = {"group_expansions": ge, "leaves": lv, "plane_tests": pl, "inside_tests": ins,
+                   "live_children": live, "slab_tests": slab,
                    "bytes": K_RAY + K_OUT + K_GROUP * ge + K_POP * (ge - 1 + lv) + K_LEAF * lv + K_PLANE * pl
                    + K_INSIDE * ins,
-                   "ops": KOPS_RAY + KOPS_GROUP * ge + KOPS_PLANE * pl + KOPS_INSIDE * ins}
+                   "ops": KOPS_RAY + KOPS_OCTANT * live + KOPS_SLAB * slab + KOPS_PLANE * pl + KOPS_INSIDE * ins}
     log(f"kernel work per ray: {kernel_work}")
     kA_ms, wallA, outA = timed(rA, args.steps)
     log(f"batch A: {kA_ms:.3f} ms/launch ({len(coherent) / kA_ms / 1e3:.1f} Mrays/s)")

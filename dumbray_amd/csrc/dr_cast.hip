@@ -34,6 +34,7 @@ constexpr int kLeafMin = 16;          // pending leaves that trigger the (wave-s
 // wave needs a slot for a new range while both hold unfinished ranges, the older one
 // is written out as far as it is done, and its remaining rays store their results
 // directly.
+constexpr int kCastCounters = 6;     // per-ray counters of dr_cast_stats
 constexpr int kSlots = 2;
 constexpr int kPendingId = -2;        // staged entry of a ray still being traced
 
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(kCastBlock, 4) void k_cast(DevTree T, const float *
   bool exhausted = false;
   Walk w;
   w.state = kWalkDone;
-  CastCounters cnt_local = {0, 0, 0, 0};
+  CastCounters cnt_local = {0, 0, 0, 0, 0, 0};
   CastCounters *cc = kStats ? &cnt_local : nullptr;
   auto finish = [&]() {
     const int id = w.hit ? w.h.id : -1;
@@ -87,11 +88,14 @@ __global__ __launch_bounds__(kCastBlock, 4) void k_cast(DevTree T, const float *
       out_p[3 * (size_t)r + 2] = w.hit ? w.h.p.z : 0.0f;
     }
     if (kStats) {
-      stats[4 * (size_t)r + 0] = cnt_local.expand;
-      stats[4 * (size_t)r + 1] = cnt_local.leaves;
-      stats[4 * (size_t)r + 2] = cnt_local.plane;
-      stats[4 * (size_t)r + 3] = cnt_local.inside;
-      cnt_local = {0, 0, 0, 0};
+      int *o = stats + kCastCounters * (size_t)r;
+      o[0] = cnt_local.expand;
+      o[1] = cnt_local.leaves;
+      o[2] = cnt_local.plane;
+      o[3] = cnt_local.inside;
+      o[4] = cnt_local.live;
+      o[5] = cnt_local.slab;
+      cnt_local = {0, 0, 0, 0, 0, 0};
     }
     r = -1;
   };

@@ -370,15 +370,18 @@ enum : int {
   kRcPlane,             // triangle plane tests
   kRcInside,            // triangle inside tests
   kRcPaths,             // path jobs (pixel samples) started
+  kRcLive,              // children of expanded groups with a non-empty leaf below (octant tests)
+  kRcSlab,              // children whose slab test ran
   kRcCount
 };
 struct RenderCounters {
-  CastCounters cast = {0, 0, 0, 0};
+  CastCounters cast = {0, 0, 0, 0, 0, 0};
   int segments = 0, photons = 0, paths = 0;
 };
 // Adds one lane's counters to the launch's totals (every lane of the wave calls it).
 __device__ __forceinline__ void flush_counters(const RenderCounters &c, unsigned long long *out) {
-  const int v[kRcCount] = {c.segments, c.photons, c.cast.expand, c.cast.leaves, c.cast.plane, c.cast.inside, c.paths};
+  const int v[kRcCount] = {c.segments, c.photons,     c.cast.expand, c.cast.leaves, c.cast.plane,
+                           c.cast.inside, c.paths, c.cast.live, c.cast.slab};
 #pragma unroll
   for (int k = 0; k < kRcCount; k++) {
     unsigned long long x = (unsigned long long)v[k];

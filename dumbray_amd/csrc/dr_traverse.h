@@ -34,10 +34,12 @@ struct Hit {
 
 // Work counters of one cast (measurement launches only; nullptr elsewhere).
 struct CastCounters {
-  int expand;    // groups whose 8 child tests ran (196-B reads: 8 boxes + non-empty mask)
+  int expand;    // groups expanded (196-B reads: 8 boxes + non-empty mask)
   int leaves;    // leaves whose triangles were tested (+ leaf box and link/count reads)
   int plane;     // triangle records whose plane test ran (first 32 B)
   int inside;    // records that reached the inside test (second 32 B)
+  int live;      // children of expanded groups with a non-empty leaf below (octant tests run)
+  int slab;      // children whose slab test ran (live and not behind the ray's origin)
 };
 
 // A ray prepared for traversal.
@@ -132,8 +134,10 @@ constexpr int kStackWordsPerLevel = 2;
 // the ray (the thresholds of set_prune), and the reference slab test on its box passes.
 template <bool kFinite>
 __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g, const TRay &r, uint32_t &lo,
-                                            uint32_t &hi) {
+                                            uint32_t &hi, CastCounters *cc = nullptr) {
   const float4 *q = G + kGroupF4 * (size_t)g;
+  // children with no non-empty leaf below them are dropped before their box tests run
+  const uint2 head = *(const uint2 *)(q + kGroupMask / 4);
   uint32_t m = 0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {       // two halves of 4 children keep register use low
@@ -144,16 +148,16 @@ __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g,
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       // forward octant of o: coordinate k of o + d*t (t >= 0) is >= o.k if d.k >= 0, <= o.k if d.k <= 0
-      const bool behind = (a_ex[j] < r.hi.x) || (a_sx[j] > r.lo.x) || (a_ey[j] < r.hi.y) || (a_sy[j] > r.lo.y) ||
-                          (a_ez[j] < r.hi.z) || (a_sz[j] > r.lo.z);
+      const bool live = (head.x >> (4 * h + j)) & 1u;
+      const bool behind = !live || (a_ex[j] < r.hi.x) || (a_sx[j] > r.lo.x) || (a_ey[j] < r.hi.y) ||
+                          (a_sy[j] > r.lo.y) || (a_ez[j] < r.hi.z) || (a_sz[j] > r.lo.z);
+      if (cc) { cc->live += live ? 1 : 0; cc->slab += behind ? 0 : 1; }
       const bool hit =
           !behind && (kFinite ? slab_hit_finite(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv)
                               : slab_hit(a_sx[j], a_sy[j], a_sz[j], a_ex[j], a_ey[j], a_ez[j], r.o, r.inv));
       m |= hit ? (1u << (4 * h + j)) : 0u;
     }
   }
-  const uint2 head = *(const uint2 *)(q + kGroupMask / 4);
-  m &= head.x & 0xffu;
   const int c = r.canon;
   if (c & 1) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
   if (c & 2) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
@@ -239,8 +243,8 @@ __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *s
   // the NaN-free form of the slab test when no expanding lane of the wave has an infinite 1/d
   const bool finite = __builtin_isfinite(w.ray.inv.x) && __builtin_isfinite(w.ray.inv.y) &&
                       __builtin_isfinite(w.ray.inv.z);
-  if (kFastSlab && __ballot(!finite) == 0) group_entry<true>(T.groups, child, w.ray, lo2, hi2);
-  else group_entry<false>(T.groups, child, w.ray, lo2, hi2);
+  if (kFastSlab && __ballot(!finite) == 0) group_entry<true>(T.groups, child, w.ray, lo2, hi2, cc);
+  else group_entry<false>(T.groups, child, w.ray, lo2, hi2, cc);
   ++w.sp;
   stk[2 * w.sp * stride] = lo2;
   stk[(2 * w.sp + 1) * stride] = hi2;

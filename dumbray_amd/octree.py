@@ -197,14 +197,14 @@ class DeviceScene:
 
     def cast_stats(self, rays, clip_backfaces: bool = True, stream=None):
         """dr_cast_stats: ids, t and per-ray counters (groups expanded, leaves, plane tests,
-        inside tests) of the traversal the kernel ran."""
+        inside tests, live children, slab tests) of the traversal the kernel ran."""
         import torch
 
         rays = rays.contiguous().view(-1, 6)
         n = rays.shape[0]
         ids = torch.empty(n, dtype=torch.int32, device=rays.device)
         t = torch.empty(n, dtype=torch.float32, device=rays.device)
-        cnt = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        cnt = torch.empty((n, 6), dtype=torch.int32, device=rays.device)
         s = stream if stream is not None else torch.cuda.current_stream(rays.device)
         rc = lib().dr_cast_stats(self._h, _vp(rays.data_ptr()), int(n),
                                  DR_CAST_CLIP_BACKFACES if clip_backfaces else 0, None, _vp(ids.data_ptr()),

@@ -291,7 +291,8 @@ class DumbRenderer:
         check(lib().dr_render_blocks(self.scene.handle, ctypes.byref(p), _vp(frame.data.data_ptr()), int(start_block),
                                      int(end_block), _vp(s.cuda_stream)), "dr_render_blocks")
 
-    RENDER_COUNTERS = ("segments", "photons", "group_expansions", "leaves", "plane_tests", "inside_tests", "paths")
+    RENDER_COUNTERS = ("segments", "photons", "group_expansions", "leaves", "plane_tests", "inside_tests", "paths",
+                       "live_children", "slab_tests")
 
     def render_stats(self, frame: BlockBasedFrameBuffer | None = None, iteration: int = 1, stream=None):
         """dr_render_stats: renders like render() and returns (frame, counters of the work the

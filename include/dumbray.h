@@ -378,17 +378,19 @@ int dr_png_read(const char *path, uint8_t *rgba, int width, int height);
 
 /* dr_cast plus per-ray work counters of the traversal that ran (measurement only;
  * no counterpart in the reference, which prints timings only): out_counts is a
- * device int32[n*4] receiving, per ray, the groups expanded (8 child box tests,
- * 196 B read each), leaves tested, triangle plane tests (32 B record reads) and
- * inside tests (a further 32 B each).  Same results as dr_cast. */
+ * device int32[n*6] receiving, per ray, the groups expanded (196 B read each), leaves
+ * tested, triangle plane tests (32 B record reads), inside tests (a further 32 B each),
+ * children of the expanded groups with a non-empty leaf below (octant tests) and children
+ * whose slab test ran.  Same results as dr_cast. */
 int dr_cast_stats(dr_scene s, const float *rays, int n, unsigned flags, const int32_t *exclude, int32_t *out_id,
                   float *out_t, int32_t *out_counts, void *stream);
 
 /* dr_render_blocks plus work counters of the render (measurement only; no counterpart in the
- * reference, which prints timings only).  counts: device uint64[7], added to (the caller zeroes
- * it): casts (camera, bounce and photon rays), of which light photons, group expansions (8
- * child box tests, 196 B read each), leaves tested, triangle plane tests and inside tests (32 B
- * of a record each), and path jobs started.  Same frame as dr_render_blocks. */
+ * reference, which prints timings only).  counts: device uint64[9], added to (the caller zeroes
+ * it): casts (camera, bounce and photon rays), of which light photons, group expansions (196 B
+ * read each), leaves tested, triangle plane tests and inside tests (32 B of a record each), path
+ * jobs started, children of expanded groups with a non-empty leaf below (octant tests) and
+ * children whose slab test ran.  Same frame as dr_render_blocks. */
 int dr_render_stats(dr_scene s, const dr_render_params *p, float *frame, int start_block, int end_block,
                     uint64_t *counts, void *stream);
 
