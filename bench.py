@@ -90,17 +90,19 @@ def _read_counters(outdir):
     return res
 
 
-def measure_counters(steps=3):
+def measure_counters(steps=3, keep_dir=None):
     """Fabric bytes, L2 hit rate, VALU busy and VALU lane utilization of the cast kernel on the
     incoherent batch, from rocprofv3 counter passes run by this invocation on this GPU; None
-    when rocprofv3 is missing or a pass fails."""
+    when rocprofv3 is missing or a pass fails.  With keep_dir the passes' CSV files stay there
+    (one sub-directory per pass); otherwise they go to a temporary directory that is removed."""
     import shutil
     import subprocess
     import tempfile
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
         return None
-    base = tempfile.mkdtemp(prefix="bench_pmc_", dir="/tmp")
+    base = os.path.abspath(keep_dir) if keep_dir else tempfile.mkdtemp(prefix="bench_pmc_", dir="/tmp")
+    os.makedirs(base, exist_ok=True)
     env = dict(os.environ, TMPDIR="/tmp")
     per = {}
     for tag, counters in PMC_PASSES:
@@ -116,7 +118,8 @@ def measure_counters(steps=3):
             return None
         for k, v in data.items():
             per.setdefault(k, {}).update(v)
-    shutil.rmtree(base, ignore_errors=True)
+    if not keep_dir:
+        shutil.rmtree(base, ignore_errors=True)
     cast = next((v for k, v in per.items() if CAST_KERNEL in k), None)
     if not cast:
         return None
@@ -408,7 +411,10 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing and adjoint legs")
     ap.add_argument("--adam-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-requests leg")
+    ap.add_argument("--legs", default="path_trace,rmse,small,adjoint,batch,example",
+                    help="render legs to run (comma list of path_trace, rmse, small, adjoint, batch, example)")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter passes of the cast kernel")
+    ap.add_argument("--pmc-dir", default=None, help="keep the counter passes' CSV files in this directory")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -527,7 +533,7 @@ def main():
     t = kB_ms / 1e3
     ops = kernel_work["ops"] * n
     logical = kernel_work["bytes"] * n
-    pmc = None if args.no_pmc else measure_counters()
+    pmc = None if args.no_pmc else measure_counters(keep_dir=args.pmc_dir)
     if pmc is not None:
         traffic, src = pmc["fetch_bytes"] + pmc["write_bytes"], "rocprofv3 --pmc passes launched by this bench run"
     else:
@@ -565,13 +571,21 @@ def main():
 
     render = None
     if not args.no_render:
-        render = {"path_trace_1080p": bench_path_trace(D, S, stream, args.cpu_seconds, not args.no_cpu),
-                  "rmse_vs_fp64_4096spp": bench_rmse(D, S, stream),
-                  "path_trace_256": bench_small(D, S, stream),
-                  "adjoint_512": bench_adjoint(D, S, stream, args.adam_steps),
-                  "batch_64_requests": None if args.no_batch else bench_batch(D, S, stream),
-                  "scene_file_example_0": bench_scene_file(D, stream, not args.no_cpu, args.cpu_seconds)}
+        legs = set(args.legs.split(","))
+        if args.no_batch:
+            legs.discard("batch")
+        run = lambda leg, fn: fn() if leg in legs else None
+        render = {"path_trace_1080p": run("path_trace", lambda: bench_path_trace(D, S, stream, args.cpu_seconds,
+                                                                                 not args.no_cpu)),
+                  "rmse_vs_fp64_4096spp": run("rmse", lambda: bench_rmse(D, S, stream)),
+                  "path_trace_256": run("small", lambda: bench_small(D, S, stream)),
+                  "adjoint_512": run("adjoint", lambda: bench_adjoint(D, S, stream, args.adam_steps)),
+                  "batch_64_requests": run("batch", lambda: bench_batch(D, S, stream)),
+                  "scene_file_example_0": run("example", lambda: bench_scene_file(D, stream, not args.no_cpu,
+                                                                                  args.cpu_seconds))}
 
+    pt = render["path_trace_1080p"] if render else None
+    adj = render["adjoint_512"] if render else None
     line = {
         "metric": "closest-hit Mrays/s & Mpix·spp/s at 1080p; adjoint/forward time ratio",
         "value": round(value, 2),
@@ -599,8 +613,8 @@ def main():
         "host": {"cores_used": HOST_THREADS, "nproc": os.cpu_count()},
         "oracle_sample_mismatches": parity,
         "cpu_baseline": cpu,
-        "mpix_spp_s_1080p": render["path_trace_1080p"]["mpix_spp_s"] if render else None,
-        "adjoint_forward_ratio": render["adjoint_512"]["ratio"] if render else None,
+        "mpix_spp_s_1080p": pt["mpix_spp_s"] if pt else None,
+        "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "render": render,
     }
     if cpu:

@@ -44,7 +44,7 @@ struct dr_scene_s {
   dr::DevBuf faces;                        // per triangle: vertices a, b, c, then their normals (18 floats)
   std::vector<float> faces_host;
   dr::DevTree tree;
-  int max_depth = 0;
+  int max_depth = 0;                       // depth the octree reached: traversal stack levels
   int ntris = 0;
   bool has_materials = false;
   int max_mat = -1;                        // largest material id in use

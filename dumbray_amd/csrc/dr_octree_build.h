@@ -59,7 +59,8 @@ struct OctreeFlat {
   std::vector<int32_t> lref;               // leaf entry -> its record in urecs
   Box root;
   int root_is_leaf = 0, root_start = 0, root_count = 0;
-  int ngroups = 0, nrec = 0, max_depth = 0;
+  int ngroups = 0, nrec = 0;
+  int max_depth = 0;                       // depth of the deepest node (root: 0), not the build limit
   float prune_margin = 0.0f;               // see the group layout above
 };
 int flatten_octree(const OctreeHost &t, OctreeFlat &out);

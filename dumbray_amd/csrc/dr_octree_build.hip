@@ -508,7 +508,17 @@ int flatten_octree(const OctreeHost &t, OctreeFlat &out) {
   const int nn = (int)t.bounds.size();
   if (nn < 1) return -1;
   out.root = t.bounds[0];
-  out.max_depth = t.max_depth;
+  // deepest node the build reached (breadth-first order: parents precede their children);
+  // a walk holds one stack level per interior node on its path, so this many levels suffice
+  {
+    std::vector<int> depth(nn, 0);
+    for (int i = 0; i < nn; i++)
+      if (t.children[i] >= 0)
+        for (int c = t.children[i]; c < t.children[i] + 8; c++) {
+          depth[c] = depth[i] + 1;
+          out.max_depth = std::max(out.max_depth, depth[c]);
+        }
+  }
   auto nondegenerate = [&](int32_t id) {
     const Tri &tr = t.tris[id];
     return !(eq(tr.a, tr.b) || eq(tr.b, tr.c) || eq(tr.c, tr.a));    // degenerate never hits (Triangle.impl.h:292)

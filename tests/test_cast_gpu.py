@@ -87,6 +87,33 @@ def test_gpu_cast_knobs(dev, oracle, thresh, depth):
     _assert_same(g, o)
 
 
+def _reached_depth(tris, thresh, depth):
+    t = D.Octree(thresh, depth)
+    t.push(tris)
+    t.build()
+    _, ch, _, _ = t.export()
+    d = np.zeros(len(ch), np.int64)
+    for i in range(len(ch)):             # breadth-first order: parents precede their children
+        if ch[i] >= 0:
+            d[ch[i]:ch[i] + 8] = d[i] + 1
+    return int(d.max())
+
+
+def test_gpu_cast_stack_sized_by_reached_depth(dev, oracle):
+    """The traversal stack holds one level per depth the build reached, not per level the
+    depth limit allows: trees that stop far above their limit of 24 (a 32,768-tri
+    heightfield split down to single triangles, depth 11; a sparse soup, depth 4) give the
+    oracle's hits."""
+    cases = ((S.heightfield(128, seed=1), 1, 11), (S.random_soup(2500, seed=7, extent=5.0, size=1.5), 8, 4))
+    for tris, thresh, reached in cases:
+        assert _reached_depth(tris, thresh, 24) == reached
+        rays = _rays(tris, 8000, seed=21)
+        print(f"[case] reached-depth stack: tris={len(tris)} thresh={thresh} reached={reached}", flush=True)
+        ref = oracle.Octree(tris, thresh, 24).cast(rays, points=True)
+        assert (ref["id"] >= 0).mean() > 0.1
+        _assert_same(_gpu_cast(tris, rays, False, thresh, 24), ref)
+
+
 def test_gpu_cast_edge_cases(dev, oracle):
     import torch
     # axis-aligned rays (zero direction components -> infinite inverse), rays through
