@@ -266,14 +266,15 @@ int dr_backward(dr_scene s, const dr_render_params *p, const float *dl_dframe, d
 
 /* dr_backward for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
- * reference.  Same arguments and results as dr_backward; the vertex records of the path
- * replay live in a 40 MiB array that the library holds on every device it runs on until
+ * reference.  Same arguments and results as dr_backward; the block counter and the vertex
+ * records of the path replay live in a 40 MiB array that the library holds on every device it runs on until
  * the process ends (its address is resolved by dr_device_open, or by the first call on a
  * device), and the persistent grid is sized to what that array holds (all resident
  * workgroups up to max_bounces 4, fewer beyond).  Calls no hipMalloc/hipFree.  Every call
  * on one device shares the array: calls that overlap on a device (several streams or host
  * threads) race and must be serialised by the caller.  Returns DR_ERR_ARG when one
- * workgroup's records (256 lanes x (max_bounces + 1) x 36 B) exceed the array. */
+ * workgroup's records (256 lanes x (max_bounces + 1) x 36 B) and the 256-B counter exceed
+ * the array. */
 int dr_backward_noalloc(dr_scene s, const dr_render_params *p, const float *dl_dframe, double *g_albedo,
                         double *g_emission, void *stream);
 
