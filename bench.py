@@ -537,8 +537,8 @@ def main():
     if pmc is not None:
         traffic, src = pmc["fetch_bytes"] + pmc["write_bytes"], "rocprofv3 --pmc passes launched by this bench run"
     else:
-        traffic = read_traffic(os.path.join(ROOT, "profiles", "cast_incoherent_hf2M_traffic.txt"))
-        src = "profiles/cast_incoherent_hf2M_traffic.txt (committed; the counter passes did not run)"
+        traffic = read_traffic(os.path.join(ROOT, "profiles", "cast_incoherent_hf2M_counters.txt"))
+        src = "profiles/cast_incoherent_hf2M_counters.txt (committed; the counter passes did not run)"
     roofline = {"bound": "valu", "achieved": round(ops / t / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
                 "unit": "T lane-ops/s", "frac": round(ops / t / 1e12 / VALU_PEAK_TOPS, 4),
                 "traffic": None if traffic is None else int(traffic),
