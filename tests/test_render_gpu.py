@@ -64,6 +64,20 @@ def test_render_specular_rr_two_lights_matches_oracle(dev, oracle):
     _assert_close(img, ref)
 
 
+@pytest.mark.parametrize("spp", [(17, 16), (1, 1)])
+def test_sample_counts_outside_the_streamed_kernel_and_at_one(dev, oracle, spp):
+    """272 spp is past the streamed kernel's limit (256): the launch takes the one-block-per-
+    workgroup kernel with its staging rounds; 1 spp takes the streamed kernel with its largest
+    ring of pixel slots.  Both against the oracle."""
+    sc = S.spheres_scene(8, seed=0)
+    print(f"[case] spheres 24x20 spp {spp} depth 3", flush=True)
+    g, o = _pair(oracle, sc, 24, 20, spp, 3)
+    _, img = _gpu_image(g)
+    ref = o.render()
+    assert ref.max() > 0
+    _assert_close(img, ref)
+
+
 def test_progressive_blend_and_block_range(dev, oracle):
     import torch
     sc = S.spheres_scene(8, seed=1)
