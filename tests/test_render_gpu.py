@@ -78,6 +78,44 @@ def test_sample_counts_outside_the_streamed_kernel_and_at_one(dev, oracle, spp):
     _assert_close(img, ref)
 
 
+def test_concurrent_renders_and_casts_on_streams(dev):
+    """Renders in flight on several streams at once, with casts of the same scene between
+    them: every launch takes its own block counter from the scene's ring of counter pairs,
+    so each frame equals the same render issued alone, over enough launches to wrap the
+    ring (1024 pairs)."""
+    import torch
+    sc = S.spheres_scene(8, seed=4)
+    tree = D.Octree(16, 12)
+    tree.push(sc.tris)
+    tree.build()
+    ds = tree.upload()
+    shapes = [(40, 24, (1, 1)), (64, 48, (2, 2)), (33, 70, (4, 4)), (96, 80, (1, 2))]
+    gs = [D.DumbRenderer(ds, sc.mat_id, sc.materials, sc.lights, sc.camera, w, h, spp, 3) for w, h, spp in shapes]
+    want = []
+    for g in gs:
+        f = g.render()
+        torch.cuda.synchronize()
+        want.append(f.data.clone())
+    rays = torch.from_numpy(S.incoherent_rays(sc.tris, 3000, seed=9)).cuda()
+    hits = ds.cast(rays, clip_backfaces=False)["id"].clone()
+    streams = [torch.cuda.Stream() for _ in gs]
+    frames = [D.BlockBasedFrameBuffer(g.width, g.height) for g in gs]
+    print(f"[case] concurrent renders on {len(streams)} streams, 280 rounds", flush=True)
+    torch.cuda.synchronize()
+    for rnd in range(280):
+        outs = []
+        for g, f, st in zip(gs, frames, streams):
+            with torch.cuda.stream(st):
+                g.renderBlocksGPU(f, 0, f.getBlockCount(), 1, st)
+                outs.append(ds.cast(rays, clip_backfaces=False, stream=st)["id"])
+        torch.cuda.synchronize()
+        if rnd % 40 == 0 or rnd == 279:
+            for f, ref in zip(frames, want):
+                assert torch.equal(f.data.view(torch.int32), ref.view(torch.int32)), rnd
+            for o in outs:
+                assert torch.equal(o, hits), rnd
+
+
 def test_progressive_blend_and_block_range(dev, oracle):
     import torch
     sc = S.spheres_scene(8, seed=1)
