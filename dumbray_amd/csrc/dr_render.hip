@@ -24,7 +24,6 @@ constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per ro
 constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
 constexpr int kLeafMin = 16;           // pending leaves that trigger the (wave-shared) leaf tests
 constexpr int kShadeMin = 32;          // finished casts that trigger the path updates
-constexpr int kMaxRenderGroup = 8;     // framebuffer blocks one workgroup renders together (low spp)
 // Idle lanes a wave of k_render collects before it starts new path jobs (1, 8, 16, 32
 // measured 34.9 / 35.0 / 35.3 / 38.4 ms at 1080p 16 spp, 0.84 / 0.77 / 0.80 / 0.90 ms at
 // 256x256 4 spp, 15.7 / 15.3 / 15.6 / 16.3 ms for the example scene at 1 spp).
@@ -417,19 +416,16 @@ __device__ __forceinline__ void store_pixel(const dr_render_params &P, float *__
   }
 }
 
-// Renders framebuffer blocks [block, block + nb) of one request (all lanes of the
-// workgroup call it).  nb > 1 only when every sample of the nb blocks fits one round
-// of the LDS staging (nb * samples per pixel <= kRoundSamples): at low sample counts
-// one block has fewer paths than the workgroup has lanes.
+// Renders framebuffer block `block` of one request (all lanes of the workgroup call it),
+// in rounds of kRoundSamples samples per pixel.
 // refill_min: an idle lane takes a new job once at least this many lanes of its wave
 // are idle (or the whole wave is).
 template <class PathT, bool kStats = false>
 __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb, const dr_render_params &P,
-                                             float *__restrict__ frame, int block, int nb, const BlockLds &lds,
+                                             float *__restrict__ frame, int block, const BlockLds &lds,
                                              int refill_min, RenderCounters *rc = nullptr) {
   CastCounters *cc = kStats ? &rc->cast : nullptr;
   const int spp = P.fsaa_x * P.fsaa_y;
-  const int npix = kBlockPixels * nb;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
   const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
   const bool clip = P.cull_backfaces != 0;
@@ -439,8 +435,8 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
   for (int t = threadIdx.x; t < kBlockPixels * 3; t += kRB) lds.acc[t] = 0.0f;
 
   for (int s_base = 0; s_base < spp; s_base += kRoundSamples) {
-    const int ns = min(kRoundSamples / nb, spp - s_base);
-    const int njobs = npix * ns;       // job j: pixel j / ns of the run, sample s_base + j % ns
+    const int ns = min(kRoundSamples, spp - s_base);
+    const int njobs = kBlockPixels * ns;   // job j: pixel j / ns, sample s_base + j % ns
     if (threadIdx.x == 0) *lds.next_job = 0;
     __syncthreads();
     PathT ps;
@@ -471,9 +467,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
           if (slot < njobs) {
             job = slot;
             const int rp = slot / ns, s = s_base + (slot - rp * ns);
-            const bool ok = (nb == 1) ? ps.start(P, block, rp, s, samp0)
-                                      : ps.start(P, block + rp / kBlockPixels, rp % kBlockPixels, s, samp0);
-            if (!ok) {
+            if (!ps.start(P, block, rp, s, samp0)) {
               const float zero[3] = {0.0f, 0.0f, 0.0f};
               finish(zero);
             } else {
@@ -510,32 +504,20 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
       }
     }
     __syncthreads();
-    if (nb == 1) {
-      if (threadIdx.x < kBlockPixels) {
-        for (int s = 0; s < ns; s++)
-          for (int c = 0; c < 3; c++) lds.acc[3 * threadIdx.x + c] += samp_col[3 * (threadIdx.x * ns + s) + c] * spw;
-      }
-    } else {
-      // the only round: the pixel sums (same order as the accumulation above) go out at once
-      for (int rp = threadIdx.x; rp < npix; rp += kRB) {
-        float a[3] = {0.0f, 0.0f, 0.0f};
-        for (int s = 0; s < ns; s++)
-          for (int c = 0; c < 3; c++) a[c] += samp_col[3 * (rp * ns + s) + c] * spw;
-        store_pixel(P, frame, block + rp / kBlockPixels, rp % kBlockPixels, a);
-      }
+    if (threadIdx.x < kBlockPixels) {
+      for (int s = 0; s < ns; s++)
+        for (int c = 0; c < 3; c++) lds.acc[3 * threadIdx.x + c] += samp_col[3 * (threadIdx.x * ns + s) + c] * spw;
     }
     __syncthreads();
   }
-  if (nb == 1 && threadIdx.x < kBlockPixels) store_pixel(P, frame, block, threadIdx.x, lds.acc + 3 * threadIdx.x);
+  if (threadIdx.x < kBlockPixels) store_pixel(P, frame, block, threadIdx.x, lds.acc + 3 * threadIdx.x);
   __syncthreads();
 }
 
-// kGrouped: workgroups take runs of `group` framebuffer blocks (low sample counts);
-// otherwise one block each, with the run length fixed at 1 at compile time.
-template <class PathT, bool kGrouped, bool kStats = false>
+// One framebuffer block per workgroup (sample counts above the streamed kernel's limit).
+template <class PathT, bool kStats = false>
 __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables tb, dr_render_params P,
-                                                               float *__restrict__ frame, int start_block,
-                                                               int end_block, int group, int levels,
+                                                               float *__restrict__ frame, int start_block, int levels,
                                                                unsigned long long *__restrict__ counts = nullptr) {
   extern __shared__ uint32_t lds[];
   __shared__ float acc[kBlockPixels * 3];
@@ -544,10 +526,8 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
   __shared__ int ins[kStats ? kRB : 1];
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
                       best + (threadIdx.x & ~63u), kStats ? ins + (threadIdx.x & ~63u) : nullptr};
-  const int first = start_block + (int)blockIdx.x * group;
   RenderCounters rc;
-  render_block<PathT, kStats>(T, tb, P, frame, first, kGrouped ? min(group, end_block - first) : 1, L,
-                              kRenderRefillMin, &rc);
+  render_block<PathT, kStats>(T, tb, P, frame, start_block + (int)blockIdx.x, L, kRenderRefillMin, &rc);
   if (kStats) flush_counters(rc, counts);
 }
 
@@ -1046,7 +1026,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
     const int4 w = items[it];
     const DevRequest &R = reqs[w.x];
     const Tables tb = {R.shade, R.faces};
-    for (int b = w.y; b < w.y + w.z; b++) render_block<PathT>(R.T, tb, R.P, R.frame, b, 1, L, refill_min);
+    for (int b = w.y; b < w.y + w.z; b++) render_block<PathT>(R.T, tb, R.P, R.frame, b, L, refill_min);
   }
 }
 
@@ -1072,16 +1052,6 @@ static int check_params(dr_scene s, const dr_render_params *p) {
   return DR_OK;
 }
 
-// Framebuffer blocks per workgroup: at low sample counts a block has fewer paths than a
-// workgroup has lanes, so a workgroup takes a run of blocks (within one round of the
-// sample staging, at most kMaxRenderGroup) while the launch keeps at least
-// kRenderGroupsPerCu workgroups per CU.
-constexpr int kRenderGroupsPerCu = 4;
-static int render_group(int spp, int nblocks, int num_cu) {
-  const int fill = nblocks / (std::max(num_cu, 1) * kRenderGroupsPerCu);
-  return std::max(1, std::min({kMaxRenderGroup, kRoundSamples / spp, fill}));
-}
-
 static int block_count(const dr_render_params *p) { return ((p->width + 7) / 8) * ((p->height + 15) / 16); }
 
 // The shading tables a render reads: its dr_shading handle's, or the scene's own.
@@ -1104,7 +1074,8 @@ static int launch_render(dr_scene s, const dr_render_params *p, float *frame, in
   const Tables tb = tables_of(s, p);
   const bool ref = p->plugins == DR_PLUGINS_REFERENCE;
   const int spp = p->fsaa_x * p->fsaa_y;
-  // streamed blocks (k_render_stream) unless the sample count or the stream length is too large
+  // streamed blocks (k_render_stream) when each workgroup gets at least two blocks, unless the
+  // sample count or the stream length is too large; one block per workgroup otherwise
   const long long nrange = end_block - start_block;
   if (spp <= kStreamMaxSpp && (nrange + 1) * kBlockPixels * spp < (1ll << 31) && end_block < (1 << 23)) {
     const int nslots = stream_slots(spp);
@@ -1116,22 +1087,23 @@ static int launch_render(dr_scene s, const dr_render_params *p, float *frame, in
       DR_HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
     int per_cu = 0;
     DR_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kRB, lds_s));
-    const int groups = (int)std::min<long long>(nrange, (long long)s->num_cu * std::max(per_cu, 1));
-    unsigned *work = (unsigned *)s->work.p + 2 * (s->work_slot.fetch_add(1) % kWorkSlots);
-    hipLaunchKernelGGL(kern, dim3(groups), dim3(kRB), lds_s, (hipStream_t)stream, s->tree, tb, *p, frame, start_block,
-                       end_block, levels, work, counts);
-    DR_HIP_CHECK(hipGetLastError());
-    return DR_OK;
+    const long long resident = (long long)s->num_cu * std::max(per_cu, 1);
+    const int groups = (int)std::min<long long>(nrange, resident);
+    if (nrange >= 2 * resident) {
+      unsigned *work = (unsigned *)s->work.p + 2 * (s->work_slot.fetch_add(1) % kWorkSlots);
+      hipLaunchKernelGGL(kern, dim3(groups), dim3(kRB), lds_s, (hipStream_t)stream, s->tree, tb, *p, frame,
+                         start_block, end_block, levels, work, counts);
+      DR_HIP_CHECK(hipGetLastError());
+      return DR_OK;
+    }
   }
   const size_t lds = ((size_t)levels * kStackWordsPerLevel * kRB + (size_t)kBlockPixels * kRoundSamples * 3) * 4;
-  const int group = render_group(spp, end_block - start_block, s->num_cu);
-  const int groups = (end_block - start_block + group - 1) / group;
-  auto kernel = counts ? (ref ? (group > 1 ? k_render<RefPath, true, true> : k_render<RefPath, false, true>)
-                              : (group > 1 ? k_render<BenchPath, true, true> : k_render<BenchPath, false, true>))
-                       : (ref ? (group > 1 ? k_render<RefPath, true> : k_render<RefPath, false>)
-                              : (group > 1 ? k_render<BenchPath, true> : k_render<BenchPath, false>));
-  hipLaunchKernelGGL(kernel, dim3(groups), dim3(kRB), lds, (hipStream_t)stream, s->tree, tb, *p, frame, start_block,
-                     end_block, group, levels, counts);
+  auto kernel = counts ? (ref ? k_render<RefPath, true> : k_render<BenchPath, true>)
+                       : (ref ? k_render<RefPath, false> : k_render<BenchPath, false>);
+  if (lds > 64 * 1024)
+    DR_HIP_CHECK(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kernel, dim3(end_block - start_block), dim3(kRB), lds, (hipStream_t)stream, s->tree, tb, *p,
+                     frame, start_block, levels, counts);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
 }

@@ -66,9 +66,10 @@ def test_render_specular_rr_two_lights_matches_oracle(dev, oracle):
 
 @pytest.mark.parametrize("spp", [(17, 16), (1, 1)])
 def test_sample_counts_outside_the_streamed_kernel_and_at_one(dev, oracle, spp):
-    """272 spp is past the streamed kernel's limit (256): the launch takes the one-block-per-
-    workgroup kernel with its staging rounds; 1 spp takes the streamed kernel with its largest
-    ring of pixel slots.  Both against the oracle."""
+    """272 spp is past the streamed kernel's limit (256): the one-block-per-workgroup kernel
+    stages it in 17 rounds of 16 samples.  1 spp on a frame of few blocks also takes that
+    kernel, with one round.  Both against the oracle (the streamed kernel at 1-4 spp is
+    checked on large frames in test_render_runs_gpu.py)."""
     sc = S.spheres_scene(8, seed=0)
     print(f"[case] spheres 24x20 spp {spp} depth 3", flush=True)
     g, o = _pair(oracle, sc, 24, 20, spp, 3)
@@ -79,17 +80,17 @@ def test_sample_counts_outside_the_streamed_kernel_and_at_one(dev, oracle, spp):
 
 
 def test_concurrent_renders_and_casts_on_streams(dev):
-    """Renders in flight on several streams at once, with casts of the same scene between
-    them: every launch takes its own block counter from the scene's ring of counter pairs,
-    so each frame equals the same render issued alone, over enough launches to wrap the
-    ring (1024 pairs)."""
+    """Streamed renders (frames of at least two blocks per resident workgroup) in flight on
+    several streams at once, with casts of the same scene between them: every launch takes
+    its own block counter from the scene's ring of counter pairs, so each frame equals the
+    same render issued alone, over enough launches to wrap the ring (1024 pairs)."""
     import torch
     sc = S.spheres_scene(8, seed=4)
     tree = D.Octree(16, 12)
     tree.push(sc.tris)
     tree.build()
     ds = tree.upload()
-    shapes = [(40, 24, (1, 1)), (64, 48, (2, 2)), (33, 70, (4, 4)), (96, 80, (1, 2))]
+    shapes = [(512, 400, (1, 1)), (520, 410, (2, 1)), (640, 320, (1, 1)), (600, 420, (1, 2))]
     gs = [D.DumbRenderer(ds, sc.mat_id, sc.materials, sc.lights, sc.camera, w, h, spp, 3) for w, h, spp in shapes]
     want = []
     for g in gs:
@@ -100,6 +101,8 @@ def test_concurrent_renders_and_casts_on_streams(dev):
     hits = ds.cast(rays, clip_backfaces=False)["id"].clone()
     streams = [torch.cuda.Stream() for _ in gs]
     frames = [D.BlockBasedFrameBuffer(g.width, g.height) for g in gs]
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    assert all(f.getBlockCount() >= 2 * 3 * ncu for f in frames)      # the streamed kernel
     print(f"[case] concurrent renders on {len(streams)} streams, 280 rounds", flush=True)
     torch.cuda.synchronize()
     for rnd in range(280):

@@ -1,11 +1,13 @@
-"""Large frames at low sample counts, where dr_render_blocks gives each workgroup a run
-of up to 8 consecutive framebuffer blocks (run length min(8, 16 / spp, blocks / (4 CUs))).
+"""Large frames at low sample counts, where dr_render_blocks streams many framebuffer
+blocks through each persistent workgroup (8,125 blocks over at most a few hundred
+workgroups: the ring of block ids wraps many times and workgroups end on different
+blocks, and at 1 spp the ring of pixel slots spans 8 blocks).
 
-The frame must be bit-identical to the same frame rendered through block ranges small
-enough for one block per workgroup, for both plugin sets, with partial blocks at the
-right and bottom edges and run lengths that do not divide the block count, and on a
-second iteration (the 1/k blend).  A band of rows must agree with the oracle within
-|gpu - oracle| <= 1e-4 * |oracle| + 1e-6 (the tolerance of the other render tests)."""
+The frame must be bit-identical to the same frame rendered through ranges of 600 blocks
+(another assignment of blocks to workgroups and stream positions), for both plugin sets,
+with partial blocks at the right and bottom edges, and on a second iteration (the 1/k
+blend).  A band of rows must agree with the oracle within |gpu - oracle| <= 1e-4 * |oracle|
++ 1e-6 (the tolerance of the other render tests)."""
 import os
 
 import numpy as np
@@ -18,14 +20,8 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXAMPLE = os.path.join(HERE, "golden", "example_scene_0", "0.dumb")
 W, H = 1000, 1030            # 125 x 65 blocks; the last block column and row are partial
-CHUNK = 600                  # blocks per call for the one-block-per-workgroup frame
+CHUNK = 600                  # blocks per call for the second frame
 RTOL, ATOL = 1e-4, 1e-6
-
-
-def _run_length(spp, nblocks):
-    import torch
-    ncu = torch.cuda.get_device_properties(0).multi_processor_count
-    return max(1, min(8, 16 // spp, nblocks // (ncu * 4)))
 
 
 def _frames(g, iterations=2):
@@ -49,13 +45,10 @@ def _close(a, b):
 
 
 @pytest.mark.parametrize("spp", [(1, 1), (3, 1), (2, 2)])
-def test_benchmark_plugins_runs_equal_single_blocks(dev, oracle, spp):
+def test_benchmark_plugins_stream_equals_block_ranges(dev, oracle, spp):
     import torch
     sc = S.spheres_scene(8, seed=0)
-    n = D.render.block_count(W, H)
-    run = _run_length(spp[0] * spp[1], n)
-    print(f"[case] spheres {W}x{H} spp {spp} depth 4, run length {run}", flush=True)
-    assert run > 1
+    print(f"[case] spheres {W}x{H} spp {spp} depth 4", flush=True)
     t = D.Octree(16, 12)
     t.push(sc.tris)
     t.build()
@@ -75,14 +68,11 @@ def test_benchmark_plugins_runs_equal_single_blocks(dev, oracle, spp):
 
 
 @pytest.mark.parametrize("spp", [(1, 1), (2, 1)])
-def test_reference_plugins_runs_equal_single_blocks(dev, oracle, spp):
+def test_reference_plugins_stream_equals_block_ranges(dev, oracle, spp):
     import torch
     from oracle import scene_file as SF
     ctx = D.DumbRenderContext.fromFile(EXAMPLE)
-    n = D.render.block_count(W, H)
-    run = _run_length(spp[0] * spp[1], n)
-    print(f"[case] example scene {W}x{H} spp {spp}, run length {run}", flush=True)
-    assert run > 1
+    print(f"[case] example scene {W}x{H} spp {spp}", flush=True)
     g = ctx.renderer(W, H, samples_per_pixel=spp)
     a, b = _frames(g)
     assert torch.equal(a.data, b.data)
