@@ -1,14 +1,15 @@
 // Path tracing: DumbRenderer::renderBlocksGPU and Renderer::backward on gfx950.
 //
-// One 256-lane workgroup renders one 8x16 framebuffer block.  The block's work
-// is a list of path jobs (pixel, sample); idle lanes take the next jobs with a
-// wave64 ballot + mbcnt prefix and one LDS atomic per wave, so lanes whose
-// path ended early are regenerated instead of idling.  Each lane's current ray
-// (camera/bounce ray or light photon) is a Walk advanced by one node pop per loop
-// iteration; leaf tests and the path updates of finished casts run once enough
-// lanes of the wave need them, which keeps the wave's lanes in step.  Finished sample
-// colours are staged in LDS and summed per pixel in the reference's subpixel
-// order before the block is written (set, or blended with 1/iteration).
+// The work of a render is a list of path jobs (pixel, sample) per 8x16 framebuffer
+// block; idle lanes take the next jobs with a wave64 ballot + mbcnt prefix and one LDS
+// atomic per wave, so lanes whose path ended early are regenerated instead of idling.
+// Each lane's current ray (camera/bounce ray or light photon) is a Walk advanced by one
+// node pop per loop iteration; leaf tests and the path updates of finished casts run
+// once enough lanes of the wave need them, which keeps the wave's lanes in step.
+// Finished sample colours are staged in LDS and summed per pixel in the reference's
+// subpixel order before the pixel is written (set, or blended with 1/iteration).
+// k_render_stream runs the jobs of many blocks as one stream per persistent workgroup;
+// render_block (k_render, k_batch) runs one block per workgroup at a time.
 #include "dr_common.h"
 #include "dr_plugins.h"
 #include "dr_shade.h"
@@ -24,9 +25,10 @@ constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per ro
 constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
 constexpr int kLeafMin = 16;           // pending leaves that trigger the (wave-shared) leaf tests
 constexpr int kShadeMin = 32;          // finished casts that trigger the path updates
-// Idle lanes a wave of k_render collects before it starts new path jobs (1, 8, 16, 32
-// measured 34.9 / 35.0 / 35.3 / 38.4 ms at 1080p 16 spp, 0.84 / 0.77 / 0.80 / 0.90 ms at
-// 256x256 4 spp, 15.7 / 15.3 / 15.6 / 16.3 ms for the example scene at 1 spp).
+// Idle lanes a wave collects before it starts new path jobs (1, 8, 16, 32 measured
+// 34.9 / 35.0 / 35.3 / 38.4 ms at 1080p 16 spp, 0.84 / 0.77 / 0.80 / 0.90 ms at 256x256
+// 4 spp, 15.7 / 15.3 / 15.6 / 16.3 ms for the example scene at 1 spp, one block per
+// workgroup; 4, 8, 16 gave 29.8 / 29.8 / 30.4 ms at 1080p on the streamed kernel).
 constexpr int kRenderRefillMin = 8;
 // Waves per SIMD the render kernels are compiled for: the register cap this sets
 // (168 VGPRs) costs a few spills but beats 2 waves per SIMD clearly.
