@@ -10,6 +10,7 @@ import ctypes
 import os
 
 from ._lib import check, lib
+from .octree import launch_stream
 
 IMAGES_NO_ERROR = 0
 IMAGES_ERROR_FRAME_BUFFER_INVALID = 1
@@ -23,7 +24,7 @@ def frame_to_rgba8(frame, stream=None):
 
     w, h = frame.getSize()
     out = torch.empty((h, w, 4), dtype=torch.uint8, device=frame.data.device)
-    s = stream if stream is not None else torch.cuda.current_stream(frame.data.device)
+    s = launch_stream(stream, frame.data.device, frame.data, out)
     check(lib().dr_frame_to_rgba8(ctypes.c_void_p(frame.data.data_ptr()), int(w), int(h),
                                   ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(s.cuda_stream)), "dr_frame_to_rgba8")
     return out

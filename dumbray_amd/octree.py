@@ -16,6 +16,22 @@ import numpy as np
 
 from ._lib import DumbRayError, check, lib
 
+
+def launch_stream(stream, device, *tensors):
+    """The stream a library call launches on: torch's current stream, or `stream`, which then
+    first waits for the work queued on the current stream (where the call's input and output
+    tensors were made), and keeps those tensors' memory from reuse until its work is done."""
+    import torch
+
+    cur = torch.cuda.current_stream(device)
+    if stream is None or stream == cur:
+        return cur
+    stream.wait_stream(cur)
+    for t in tensors:
+        if t is not None:
+            t.record_stream(stream)
+    return stream
+
 OCTREE_POLYCOUNT_TO_SPLIT_NODE = 8   # Octree.cuh:11
 OCTREE_MAX_DEPTH = 24                # Octree.cuh:13
 DR_CAST_CLIP_BACKFACES = 1
@@ -185,7 +201,7 @@ class DeviceScene:
             ex = exclude.contiguous().to(device=rays.device, dtype=torch.int32).view(-1)
             if ex.numel() != n:
                 raise DumbRayError("cast: exclude must hold one triangle id per ray")
-        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        s = launch_stream(stream, rays.device, rays, ex, ids, t, pts)
         rc = lib().dr_cast(self._h, _vp(rays.data_ptr()), int(n), DR_CAST_CLIP_BACKFACES if clip_backfaces else 0,
                            _vp(ex.data_ptr()) if ex is not None else None, _vp(ids.data_ptr()), _vp(t.data_ptr()),
                            _vp(pts.data_ptr()) if pts is not None else None, _vp(s.cuda_stream))
@@ -205,7 +221,7 @@ class DeviceScene:
         ids = torch.empty(n, dtype=torch.int32, device=rays.device)
         t = torch.empty(n, dtype=torch.float32, device=rays.device)
         cnt = torch.empty((n, 6), dtype=torch.int32, device=rays.device)
-        s = stream if stream is not None else torch.cuda.current_stream(rays.device)
+        s = launch_stream(stream, rays.device, rays, ids, t, cnt)
         rc = lib().dr_cast_stats(self._h, _vp(rays.data_ptr()), int(n),
                                  DR_CAST_CLIP_BACKFACES if clip_backfaces else 0, None, _vp(ids.data_ptr()),
                                  _vp(t.data_ptr()), _vp(cnt.data_ptr()), _vp(s.cuda_stream))

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import plugins as PL
 from ._lib import DumbRayError, check, lib
-from .octree import DeviceScene
+from .octree import DeviceScene, launch_stream
 
 DR_LAMBERT, DR_MIRROR, DR_METAL = 0, 1, 2
 KINDS = PL.MATERIAL_KINDS
@@ -287,7 +287,7 @@ class DumbRenderer:
         if frame.getSize() != (self.width, self.height):
             raise DumbRayError("renderBlocksGPU: frame size differs from the renderer's")
         p = self.params(iteration)
-        s = stream if stream is not None else torch.cuda.current_stream(frame.data.device)
+        s = launch_stream(stream, frame.data.device, frame.data)
         check(lib().dr_render_blocks(self.scene.handle, ctypes.byref(p), _vp(frame.data.data_ptr()), int(start_block),
                                      int(end_block), _vp(s.cuda_stream)), "dr_render_blocks")
 
@@ -303,7 +303,7 @@ class DumbRenderer:
             frame = BlockBasedFrameBuffer(self.width, self.height, self.material_table.device)
         counts = torch.zeros(len(self.RENDER_COUNTERS), dtype=torch.int64, device=frame.data.device)
         p = self.params(iteration)
-        s = stream if stream is not None else torch.cuda.current_stream(frame.data.device)
+        s = launch_stream(stream, frame.data.device, frame.data, counts)
         check(lib().dr_render_stats(self.scene.handle, ctypes.byref(p), _vp(frame.data.data_ptr()), 0,
                                     frame.getBlockCount(), _vp(counts.data_ptr()), _vp(s.cuda_stream)),
               "dr_render_stats")
@@ -327,7 +327,7 @@ class DumbRenderer:
         ga = torch.zeros((self.nmat, 3), dtype=torch.float64, device=dl.device)
         ge = torch.zeros((max(self.nlights, 1), 3), dtype=torch.float64, device=dl.device)
         p = self.params(iteration)
-        s = stream if stream is not None else torch.cuda.current_stream(dl.device)
+        s = launch_stream(stream, dl.device, dl, ga, ge)
         check(lib().dr_backward(self.scene.handle, ctypes.byref(p), _vp(dl.data_ptr()), _vp(ga.data_ptr()),
                                 _vp(ge.data_ptr()), _vp(s.cuda_stream)), "dr_backward")
         return ga, ge[: self.nlights]
