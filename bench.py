@@ -44,6 +44,9 @@ KOPS_RAY, KOPS_OCTANT, KOPS_SLAB, KOPS_PLANE, KOPS_INSIDE = 6, 12, 24, 25, 140
 
 # host threads of the CPU legs: the cores this process may run on (the lease's share of the host)
 HOST_THREADS = len(os.sched_getaffinity(0))
+# octree knobs of the path-trace and batch scenes: the reference's defaults, leaf split above
+# 8 triangles and depth limit 24 (Octree.cuh:11,13); the cast workload keeps 16 / 12
+REF_KNOBS = (8, 24)
 
 
 def log(msg):
@@ -158,8 +161,8 @@ def _event_ms(fn, k, stream):
     return e0.elapsed_time(e1) / k
 
 
-def _device_scene(D, tris):
-    t = D.Octree(16, 12)
+def _device_scene(D, tris, split=16, depth=12):
+    t = D.Octree(split, depth)
     t.push(tris)
     t.build()
     return t.upload()
@@ -193,14 +196,14 @@ def bench_path_trace(D, S, stream, cpu_seconds, with_cpu):
     level-7 icospheres (983,040 tris) + ground quad, Lambert / mirror / Schlick metal, two area lights."""
     import torch
     sc = S.icosphere_scene(level=7, seed=3)
-    ds = _device_scene(D, sc.tris)
+    ds = _device_scene(D, sc.tris, *REF_KNOBS)
     W, H, spp, depth = 1920, 1080, (4, 4), 8
     g = D.DumbRenderer(ds, sc.mat_id, sc.materials, sc.lights, sc.camera, W, H, spp, depth, rr_start=3,
                        rr_survive=0.75)
     frame = D.BlockBasedFrameBuffer(W, H)
     ms = _event_ms(lambda: g.render(frame), 3, stream)
     out = {"workload": f"path trace {W}x{H}, 16 spp, depth 8, RR from 3 (survive 0.75), {len(sc.tris)} tris, "
-                       "Lambert/mirror/Schlick metal, 2 area lights",
+                       "Lambert/mirror/Schlick metal, 2 area lights, octree split 8 / depth 24",
            "ms_per_frame": round(ms, 3), "mpix_spp_s": round(W * H * 16 / ms / 1e3, 2)}
     log(f"path trace 1080p16: {ms:.2f} ms/frame, {out['mpix_spp_s']} Mpix*spp/s")
     _, cnt = g.render_stats()
@@ -208,7 +211,7 @@ def bench_path_trace(D, S, stream, cpu_seconds, with_cpu):
     log(f"path trace counters: {cnt}; roofline {out['roofline']}")
     if with_cpu:
         import oracle
-        ot = oracle.Octree(sc.tris, 16, 12)
+        ot = oracle.Octree(sc.tris, *REF_KNOBS)
         threads = HOST_THREADS
         o = oracle.Renderer(ot, sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(), W, H, spp, depth,
                             rr_start=3, rr_survive=0.75)
@@ -383,7 +386,7 @@ def bench_batch(D, S, stream):
     rendering the same requests one dr_render_blocks call at a time."""
     import torch
     specs, qs = S.batch_workload()
-    scenes = [(sc, _device_scene(D, sc.tris)) for sc in specs]
+    scenes = [(sc, _device_scene(D, sc.tris, *REF_KNOBS)) for sc in specs]
     reqs = []
     total = 0
     for q in qs:
@@ -402,7 +405,7 @@ def bench_batch(D, S, stream):
             r.renderBlocksGPU(fr, 0, fr.getBlockCount(), it, stream)
     ms_seq = _event_ms(sequential, 2, stream)
     out = {"workload": f"64 requests over 8 icosphere scenes ({sum(len(sc.tris) for sc, _ in scenes)} tris), "
-                       f"{total / 1e6:.1f} M pixel-samples", "ms_per_batch": round(ms, 2),
+                       f"{total / 1e6:.1f} M pixel-samples, octree split 8 / depth 24", "ms_per_batch": round(ms, 2),
            "mpix_spp_s": round(total / ms / 1e3, 2), "sequential_ms": round(ms_seq, 2),
            "speedup_vs_sequential": round(ms_seq / ms, 3), "tuning": batch.tuning, "create_and_tune_s": round(tune_s, 2)}
     log(f"batch: {out}")

@@ -36,19 +36,23 @@ def _close(img, ref, what):
     assert not bad.any(), (what, int(bad.sum()), float(err.max()), np.argwhere(bad)[:5].tolist())
 
 
-def _device_scene(tris):
-    t = D.Octree(16, 12)
+# octree knobs of the 1080p and batch scenes: the reference's defaults (Octree.cuh:11,13)
+REF_KNOBS = (8, 24)
+
+
+def _device_scene(tris, split=16, depth=12):
+    t = D.Octree(split, depth)
     t.push(tris)
     t.build()
     return t.upload()
 
 
-def _oracle_trees(oracle, tri_sets):
+def _oracle_trees(oracle, tri_sets, split=16, depth=12):
     """Oracle octrees of several scenes, built in parallel threads (the builds release the GIL)."""
     out = [None] * len(tri_sets)
 
     def work(k):
-        out[k] = oracle.Octree(tri_sets[k], 16, 12)
+        out[k] = oracle.Octree(tri_sets[k], split, depth)
     th = [threading.Thread(target=work, args=(k,)) for k in range(len(tri_sets))]
     for t in th:
         t.start()
@@ -62,13 +66,13 @@ def test_path_trace_1080p_band_matches_oracle(dev, oracle):
     sc = S.icosphere_scene(level=7, seed=3)
     W, H, spp, depth = 1920, 1080, (4, 4), 8
     print(f"[case] 1080p 16 spp depth 8, {len(sc.tris)} tris", flush=True)
-    g = D.DumbRenderer(_device_scene(sc.tris), sc.mat_id, sc.materials, sc.lights, sc.camera, W, H, spp, depth,
-                       rr_start=3, rr_survive=0.75)
+    g = D.DumbRenderer(_device_scene(sc.tris, *REF_KNOBS), sc.mat_id, sc.materials, sc.lights, sc.camera, W, H,
+                       spp, depth, rr_start=3, rr_survive=0.75)
     frame = g.render()
     torch.cuda.synchronize()
     img = frame.to_image()[..., :3].cpu().numpy()
-    o = oracle.Renderer(oracle.Octree(sc.tris, 16, 12), sc.mat_id, sc.materials, sc.lights, sc.camera.as_array(),
-                        W, H, spp, depth, rr_start=3, rr_survive=0.75)
+    o = oracle.Renderer(oracle.Octree(sc.tris, *REF_KNOBS), sc.mat_id, sc.materials, sc.lights,
+                        sc.camera.as_array(), W, H, spp, depth, rr_start=3, rr_survive=0.75)
     assert np.isfinite(img).all() and img.max() > 0
     lit = np.where(img.max(axis=(1, 2)) > 0)[0]          # rows with geometry in view
     for r0 in (int(lit[len(lit) // 2]) - 8, int(lit[(3 * len(lit)) // 4]) - 8):
@@ -79,7 +83,7 @@ def test_path_trace_1080p_band_matches_oracle(dev, oracle):
 
 def _batch_renderers():
     scenes, reqs = S.batch_workload()
-    dev_scenes = [_device_scene(sc.tris) for sc in scenes]
+    dev_scenes = [_device_scene(sc.tris, *REF_KNOBS) for sc in scenes]
     rs = []
     for q in reqs:
         sc = scenes[q["scene"]]
@@ -122,7 +126,7 @@ def test_batch_64_requests_equal_solo_renders_and_oracle(dev, oracle):
             assert np.all(got[~inside] == -1.0), (replay, k)
         print(f"[case] replay {replay}: 64 frames bit-identical to solo renders", flush=True)
     # one request per scene against the oracle estimator on a band of 16 rows
-    trees = _oracle_trees(oracle, [sc.tris for sc in scenes])
+    trees = _oracle_trees(oracle, [sc.tris for sc in scenes], *REF_KNOBS)
     checked = set()
     for (r, q), f in zip(rs, frames):
         if q["scene"] in checked:
