@@ -251,6 +251,28 @@ __device__ __forceinline__ void walk_step(const DevTree &T, Walk &w, uint32_t *s
   if (cc) cc->expand++;
 }
 
+// Inclusive prefix sum of v over the 64 lanes of the wave (every lane active): DPP row
+// shifts by 1, 2, 4 and 8 (a lane without a source adds 0) scan each row of 16 lanes, and
+// each row adds the totals of the rows before it, read from lanes 15, 31 and 47 into scalar
+// registers.  `total` gets the sum over all lanes.  Unlike a scan over __shfl_up, it needs
+// no per-lane address registers, which the compiler would otherwise keep live across the
+// caller's loop.
+template <int kCtrl>
+__device__ __forceinline__ int dpp_shr_or_zero(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int wave_incl_scan(int v, int &total) {
+  v += dpp_shr_or_zero<0x111>(v);   // row_shr:1
+  v += dpp_shr_or_zero<0x112>(v);   // row_shr:2
+  v += dpp_shr_or_zero<0x114>(v);   // row_shr:4
+  v += dpp_shr_or_zero<0x118>(v);   // row_shr:8
+  const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31),
+            r2 = __builtin_amdgcn_readlane(v, 47), r3 = __builtin_amdgcn_readlane(v, 63);
+  total = (r0 + r1) + (r2 + r3);
+  const int row = (int)__lane_id() >> 4;
+  return v + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+}
+
 // castInLeaf for every pending lane of the wave at once, with all 64 lanes sharing
 // the records.  Must be called by every lane of the wave (wave-uniform control flow);
 // `pending` marks the lanes whose leaf is tested; `best` (and `ins`, counting the
@@ -274,14 +296,9 @@ __device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool p
                                                unsigned long long *best, int *ins, CastCounters *cc) {
   const int lane = (int)__lane_id();
   const int cnt = pending ? w.lcnt : 0;
-  int incl = cnt;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int v = __shfl_up(incl, off);
-    if (lane >= off) incl += v;
-  }
+  int total;
+  const int incl = wave_incl_scan(cnt, total);
   const int start = incl - cnt;
-  const int total = __shfl(incl, 63);
   if (pending) __hip_atomic_store(&best[lane], ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   if (cc && pending) __hip_atomic_store(&ins[lane], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   const int link = pending ? w.llink : 0;
