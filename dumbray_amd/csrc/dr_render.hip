@@ -51,18 +51,44 @@ struct Path {
   float gl_k = 0.0f;                    // (adjoint) cos_i/pi * cos_l*A/d^2 of the pending photon
 };
 
+// A value that is the same in every lane (a function of the kernel's parameters), moved to
+// a scalar register.  Call with every lane that holds it agreeing on it.
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ UDiv uni(UDiv D) { return {uni(D.d), uni(D.m)}; }
+__device__ __forceinline__ UDiv udiv_uniform(uint32_t d) { return uni(make_udiv(d)); }
+// frame_k(P) in scalar registers, formed before a path loop over P's blocks.
+__device__ __forceinline__ FrameK frame_k_uniform(const dr_render_params &P) {
+  FrameK K = frame_k(P);
+  K.nbw = uni(K.nbw);
+  K.fy = uni(K.fy);
+  K.hw = uni(K.hw);
+  K.hh = uni(K.hh);
+  K.ps = uni(K.ps);
+  K.pdx = uni(K.pdx);
+  K.pdy = uni(K.pdy);
+  K.jx = uni(K.jx);
+  K.ox = uni(K.ox);
+  K.jy = uni(K.jy);
+  K.oy = uni(K.oy);
+  for (int k = 0; k < 4; k++) K.zc[k] = uni(K.zc[k]);
+  K.amount = uni(K.amount);
+  K.keep = uni(K.keep);
+  return K;
+}
+
 // Common job start: camera ray of sample `s` of local pixel `lp`.  Returns false
 // (and leaves the lane idle) for pixels outside the image.
-__device__ __forceinline__ bool start_path(const dr_render_params &P, int block, int lp, int s, uint32_t samp0,
-                                           Path &st) {
+__device__ __forceinline__ bool start_path(const dr_render_params &P, const FrameK &K, int block, int lp, int s,
+                                           uint32_t samp0, Path &st) {
   int x, y;
-  if (!block_pixel(P.width, P.height, block, lp, x, y)) return false;
+  if (!block_pixel(K, P.width, P.height, block, lp, x, y)) return false;
   st.pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
   st.samp = samp0 + (uint32_t)s;
-  const int i = s / P.fsaa_y, j = s - (s / P.fsaa_y) * P.fsaa_y;
+  const int i = (int)udiv((uint32_t)s, K.fy), j = s - i * P.fsaa_y;
   float u[4];
   rng4(P, st.pix, st.samp, 0, 0, u);
-  camera_ray(P, x, y, i, j, u[0], u[1], u[2], u[3], st.o, st.d);
+  camera_ray(P, K, x, y, i, j, u[0], u[1], u[2], u[3], st.o, st.d);
   st.layer = 0;
   st.phase = 0;
   st.restrict_tri = -1;
@@ -130,8 +156,9 @@ struct BenchPath {
   // 30.8 ms with the other choices, which suit RefPath)
   static constexpr bool kStreamBlackInUpdate = false, kStreamBlockInRegister = false;
   Path st;
-  __device__ __forceinline__ bool start(const dr_render_params &P, int block, int lp, int s, uint32_t samp0) {
-    return start_path(P, block, lp, s, samp0, st);
+  __device__ __forceinline__ bool start(const dr_render_params &P, const FrameK &K, int block, int lp, int s,
+                                        uint32_t samp0) {
+    return start_path(P, K, block, lp, s, samp0, st);
   }
   __device__ __forceinline__ V3 ray_o() const { return st.o; }
   __device__ __forceinline__ V3 ray_d() const { return st.d; }
@@ -200,15 +227,16 @@ struct RefPath {
   float pc[3];
   bool unit;
 
-  __device__ __forceinline__ bool start(const dr_render_params &P, int block, int lp, int s, uint32_t samp0) {
+  __device__ __forceinline__ bool start(const dr_render_params &P, const FrameK &K, int block, int lp, int s,
+                                        uint32_t samp0) {
     int x, y;
-    if (!block_pixel(P.width, P.height, block, lp, x, y)) return false;
+    if (!block_pixel(K, P.width, P.height, block, lp, x, y)) return false;
     pix = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
     samp = samp0 + (uint32_t)s;
-    const int i = s / P.fsaa_y, j = s - (s / P.fsaa_y) * P.fsaa_y;
+    const int i = (int)udiv((uint32_t)s, K.fy), j = s - i * P.fsaa_y;
     float u[4];
     rng4(P, pix, samp, 0, 0, u);
-    camera_ray(P, x, y, i, j, u[0], u[1], u[2], u[3], o, d);
+    camera_ray(P, K, x, y, i, j, u[0], u[1], u[2], u[3], o, d);
     layer = 0;
     phase = 0;
     excl = -1;
@@ -402,34 +430,34 @@ __device__ __forceinline__ void flush_counters(const RenderCounters &c, unsigned
 
 // Framebuffer pixel `lp` of block `block`: the colour of this iteration, set (iteration 1)
 // or blended with weight 1/iteration (DumbRenderer.cu:64, BlockBasedFrameBuffer.impl.cuh:125-156).
-__device__ __forceinline__ void store_pixel(const dr_render_params &P, float *__restrict__ frame, int block, int lp,
-                                            const float a[3]) {
+__device__ __forceinline__ void store_pixel(const dr_render_params &P, const FrameK &K, float *__restrict__ frame,
+                                            int block, int lp, const float a[3]) {
   int x, y;
-  if (!block_pixel(P.width, P.height, block, lp, x, y)) return;
+  if (!block_pixel(K, P.width, P.height, block, lp, x, y)) return;
   float4 *f = (float4 *)frame + (size_t)kBlockPixels * block + lp;
   const float4 col = make_float4(a[0], a[1], a[2], 1.0f);
-  const float amount = (P.iteration <= 1) ? 1.0f : (1.0f / ((float)P.iteration));
+  const float amount = K.amount;
   if (amount >= 1.0f) *f = col;
   else {
     const float4 old = *f;
-    const float ia = (1.0f - amount);
+    const float ia = K.keep;
     *f = make_float4((col.x * amount) + (old.x * ia), (col.y * amount) + (old.y * ia),
                      (col.z * amount) + (old.z * ia), (col.w * amount) + (old.w * ia));
   }
 }
 
 // Renders framebuffer block `block` of one request (all lanes of the workgroup call it),
-// in rounds of kRoundSamples samples per pixel.
+// in rounds of kRoundSamples samples per pixel.  K = frame_k_uniform(P).
 // refill_min: an idle lane takes a new job once at least this many lanes of its wave
 // are idle (or the whole wave is).
 template <class PathT, bool kStats = false>
 __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb, const dr_render_params &P,
-                                             float *__restrict__ frame, int block, const BlockLds &lds,
-                                             int refill_min, RenderCounters *rc = nullptr) {
+                                             const FrameK &K, float *__restrict__ frame, int block,
+                                             const BlockLds &lds, int refill_min, RenderCounters *rc = nullptr) {
   CastCounters *cc = kStats ? &rc->cast : nullptr;
   const int spp = P.fsaa_x * P.fsaa_y;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
-  const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
+  const float spw = uni(1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
   const bool clip = P.cull_backfaces != 0;
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -438,6 +466,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
 
   for (int s_base = 0; s_base < spp; s_base += kRoundSamples) {
     const int ns = min(kRoundSamples, spp - s_base);
+    const UDiv Dns = udiv_uniform((uint32_t)ns);
     const int njobs = kBlockPixels * ns;   // job j: pixel j / ns, sample s_base + j % ns
     if (threadIdx.x == 0) *lds.next_job = 0;
     __syncthreads();
@@ -468,8 +497,8 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
           const int slot = base + __popcll(wmask & below);
           if (slot < njobs) {
             job = slot;
-            const int rp = slot / ns, s = s_base + (slot - rp * ns);
-            if (!ps.start(P, block, rp, s, samp0)) {
+            const int rp = (int)udiv((uint32_t)slot, Dns), s = s_base + (slot - rp * ns);
+            if (!ps.start(P, K, block, rp, s, samp0)) {
               const float zero[3] = {0.0f, 0.0f, 0.0f};
               finish(zero);
             } else {
@@ -512,7 +541,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
     }
     __syncthreads();
   }
-  if (threadIdx.x < kBlockPixels) store_pixel(P, frame, block, threadIdx.x, lds.acc + 3 * threadIdx.x);
+  if (threadIdx.x < kBlockPixels) store_pixel(P, K, frame, block, threadIdx.x, lds.acc + 3 * threadIdx.x);
   __syncthreads();
 }
 
@@ -529,7 +558,8 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
                       best + (threadIdx.x & ~63u), kStats ? ins + (threadIdx.x & ~63u) : nullptr};
   RenderCounters rc;
-  render_block<PathT, kStats>(T, tb, P, frame, start_block + (int)blockIdx.x, L, kRenderRefillMin, &rc);
+  const FrameK K = frame_k_uniform(P);
+  render_block<PathT, kStats>(T, tb, P, K, frame, start_block + (int)blockIdx.x, L, kRenderRefillMin, &rc);
   if (kStats) flush_counters(rc, counts);
 }
 
@@ -597,7 +627,9 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
   CastCounters *cc = kStats ? &rc.cast : nullptr;
   const int nrange = end_block - start_block;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
-  const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
+  const float spw = uni(1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
+  const FrameK K = frame_k_uniform(P);
+  const UDiv Dspp = udiv_uniform((uint32_t)spp), Dslots = udiv_uniform((uint32_t)nslots);
   const bool clip = P.cull_backfaces != 0;
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -609,19 +641,14 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
   bool exhausted = false; // the stream has ended for this lane
   bool nopath = false;    // the job's pixel is outside the image
   int blk = 0;            // the job's block (PathT::kStreamBlockInRegister)
-  // stages this lane's sample colour; the lane that completes the pixel writes it out
+  // stages this lane's sample colour (the lens filter applied: getPixelColor, see
+  // render_block); the lane that completes the pixel writes it out
   auto finish = [&](float l0, float l1, float l2) {
-    const int u = q / spp, s = q - u * spp, slot = u % nslots;
+    const int u = (int)udiv((uint32_t)q, Dspp), s = q - u * spp, slot = u - (int)udiv((uint32_t)u, Dslots) * nslots;
     float *dst = samp + 3 * ((size_t)slot * spp + s);
-    if (P.lens == DR_SPHERICAL_SEGMENT_LENS) {
-      dst[0] = l0 * P.lens_p[4];
-      dst[1] = l1 * P.lens_p[5];
-      dst[2] = l2 * P.lens_p[6];
-    } else {
-      dst[0] = l0;
-      dst[1] = l1;
-      dst[2] = l2;
-    }
+    dst[0] = l0;
+    dst[1] = l1;
+    dst[2] = l2;
     __threadfence_block();
     if (atomicSub(&remaining[slot], 1) == 1) {
       float a[3] = {0.0f, 0.0f, 0.0f};
@@ -632,7 +659,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
       const int b = PathT::kStreamBlockInRegister
                         ? blk
                         : (*(volatile int *)&blocks[(u / kBlockPixels) & (kStreamBlockRing - 1)] >> 8) - 1;
-      store_pixel(P, frame, b, u % kBlockPixels, a);
+      store_pixel(P, K, frame, b, u % kBlockPixels, a);
       remaining[slot] = spp;
       atomicAdd(&px_done[(u / kBlockPixels) & (kStreamBlockRing - 1)], 1);
       __threadfence_block();
@@ -654,7 +681,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
     }
     // ---- jobs taken but not started: their block and their pixel slot
     if (q >= 0 && !live) {
-      const int u = q / spp, s = q - u * spp;
+      const int u = (int)udiv((uint32_t)q, Dspp), s = q - u * spp;
       const int k = u / kBlockPixels;
       const int kk = k & (kStreamBlockRing - 1);
       if (k >= *(volatile int *)&end_k) {
@@ -682,10 +709,11 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
             e = 0;
           }
         }
-        if (e != 0 && (e & 0xff) == (k & 0xff) && *(volatile int *)&owner[u % nslots] == u) {
+        if (e != 0 && (e & 0xff) == (k & 0xff) &&
+            *(volatile int *)&owner[u - (int)udiv((uint32_t)u, Dslots) * nslots] == u) {
           live = true;
           if (PathT::kStreamBlockInRegister) blk = (e >> 8) - 1;
-          nopath = !ps.start(P, (e >> 8) - 1, u % kBlockPixels, s, samp0);
+          nopath = !ps.start(P, K, (e >> 8) - 1, u % kBlockPixels, s, samp0);
           if (nopath) {
             // a pixel outside the image: a black sample
             if (PathT::kStreamBlackInUpdate) w.state = kWalkDone;
@@ -715,8 +743,15 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
       if (live && w.state == kWalkDone) {
         if ((PathT::kStreamBlackInUpdate && nopath) || ps.shade(P, tb, w)) {
           const float *col = ps.color();
+          float c0 = col[0], c1 = col[1], c2 = col[2];
+          if (P.lens == DR_SPHERICAL_SEGMENT_LENS) {
+            c0 = c0 * P.lens_p[4];
+            c1 = c1 * P.lens_p[5];
+            c2 = c2 * P.lens_p[6];
+          }
+          // (a job outside the image stages black; its pixel is never written)
           const bool black = PathT::kStreamBlackInUpdate && nopath;
-          finish(black ? 0.0f : col[0], black ? 0.0f : col[1], black ? 0.0f : col[2]);
+          finish(black ? 0.0f : c0, black ? 0.0f : c1, black ? 0.0f : c2);
         } else {
           walk_begin(T, w, ps.ray_o(), ps.ray_d(), ps.ray_excl(), stk, kRB, nullptr);
           if (kStats) { rc.segments++; rc.photons += ps.photon() ? 1 : 0; }
@@ -832,7 +867,9 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
   const int ngrad = 3 * (P.nmat + P.nlights);
   const int spp = P.fsaa_x * P.fsaa_y;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
-  const float spw = (1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
+  const float spw = uni(1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
+  const FrameK FK = frame_k_uniform(P);
+  const UDiv Dspp = udiv_uniform((uint32_t)spp);
   const bool clip = P.cull_backfaces != 0;
   const int lane = (int)__lane_id();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -886,9 +923,9 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
           const int slot = base + __popcll(wmask & below);
           if (slot < njobs) {
             st.job = slot;
-            const int lp = slot / spp, s = slot - lp * spp;
+            const int lp = (int)udiv((uint32_t)slot, Dspp), s = slot - lp * spp;
             K = 0;
-            if (!start_path(P, block, lp, s, samp0, st)) {
+            if (!start_path(P, FK, block, lp, s, samp0, st)) {
               st.job = -1;
             } else {
               const float *g = dl + 4 * ((size_t)kBlockPixels * block + lp);
@@ -1022,13 +1059,14 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(counter, 1);
     __syncthreads();
-    const int it = item;
+    const int it = __builtin_amdgcn_readfirstlane(item);   // the same in every lane: scalar loads below
     __syncthreads();
     if (it >= nitems) break;
     const int4 w = items[it];
     const DevRequest &R = reqs[w.x];
     const Tables tb = {R.shade, R.faces};
-    for (int b = w.y; b < w.y + w.z; b++) render_block<PathT>(R.T, tb, R.P, R.frame, b, L, refill_min);
+    const FrameK K = frame_k_uniform(R.P);
+    for (int b = w.y; b < w.y + w.z; b++) render_block<PathT>(R.T, tb, R.P, K, R.frame, b, L, refill_min);
   }
 }
 

@@ -106,28 +106,67 @@ DR_HD float pixel_size(float W, float H, int boxing) {
   return 1.0f / H;
 }
 
+// Division of a non-negative int by a divisor d >= 1 through its reciprocal
+// m = floor((2^32 - 1) / d): for n < 2^32, n * m / 2^32 lies in (n / d - 1, n / d], so the
+// high word of n * m is n / d or one less, and one comparison corrects it.
+struct UDiv { uint32_t d, m; };
+DR_HD UDiv make_udiv(uint32_t d) { return {d, 0xffffffffu / d}; }
+DR_HD uint32_t udiv(uint32_t n, UDiv D) {
+  const uint32_t q = (uint32_t)(((uint64_t)n * D.m) >> 32);
+  return (n - q * D.d >= D.d) ? q + 1 : q;
+}
+
+// Values of one request's camera and framebuffer mapping that depend on the parameters
+// alone.  The kernels form them once, before the path loop, and keep them in scalar
+// registers; formed inside the loop, they would be hoisted out of it into vector registers.
+struct FrameK {
+  UDiv nbw;              // framebuffer blocks per row
+  UDiv fy;               // fsaa_y: sample s is subpixel (s / fsaa_y, s % fsaa_y)
+  float hw, hh;          // W / 2, H / 2
+  float ps, pdx, pdy;    // pixel size; subpixel width and height
+  float jx, ox, jy, oy;  // lens jitter: range offx - (-offx) and start -offx (x), the same for y
+  float zc[4];           // cam[12] * cam[9], cam[10], cam[11], cam[12]
+  float amount, keep;    // blend weight of this iteration (1 / iteration) and 1 - amount
+};
+DR_HD FrameK frame_k(const dr_render_params &P) {
+  FrameK K;
+  const float W = (float)P.width, H = (float)P.height;
+  K.nbw = make_udiv((uint32_t)((P.width + 8 - 1) / 8));
+  K.fy = make_udiv((uint32_t)P.fsaa_y);
+  K.hw = W / 2.0f;
+  K.hh = H / 2.0f;
+  K.ps = pixel_size(W, H, P.boxing);
+  K.pdx = K.ps / ((float)P.fsaa_x);
+  K.pdy = K.ps / ((float)P.fsaa_y);
+  const float offx = K.pdx * 0.5f, offy = K.pdy * 0.5f;
+  K.jx = offx - (-offx);
+  K.ox = -offx;
+  K.jy = offy - (-offy);
+  K.oy = -offy;
+  for (int k = 0; k < 4; k++) K.zc[k] = P.cam[12] * P.cam[9 + k];
+  K.amount = (P.iteration <= 1) ? 1.0f : (1.0f / ((float)P.iteration));
+  K.keep = 1.0f - K.amount;
+  return K;
+}
+
 // Camera sample of subpixel (i, j) of pixel (px, py): DumbRenderer.cu:190-232 (pixel
 // size by boxing mode), SimpleStochasticLense jitter (SimpleStochasticLense.impl.cuh:11-21 with
 // DumbRand::range = u*(max-min)+min) or DefaultPerspectiveLense, Camera transform
-// (Transform.impl.h:166-168).
-DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, float u0, float u1, float u2, float u3,
-                      V3 &o, V3 &d) {
-  const float W = (float)P.width, H = (float)P.height;
-  const float ps = pixel_size(W, H, P.boxing);
-  const float pdx = ps / ((float)P.fsaa_x), pdy = ps / ((float)P.fsaa_y);
-  const float ppx = ((float)px - (W / 2.0f)) * ps;
-  const float ppy = ((H / 2.0f) - (float)py) * ps;
-  const float sx = ppx + (((float)i) + 0.5f) * pdx;
-  const float sy = ppy + (((float)j) + 0.5f) * pdy;
-  float lx, ly, lz;
+// (Transform.impl.h:166-168).  K = frame_k(P).
+DR_HD void camera_ray(const dr_render_params &P, const FrameK &K, int px, int py, int i, int j, float u0, float u1,
+                      float u2, float u3, V3 &o, V3 &d) {
+  const float ppx = ((float)px - K.hw) * K.ps;
+  const float ppy = (K.hh - (float)py) * K.ps;
+  const float sx = ppx + (((float)i) + 0.5f) * K.pdx;
+  const float sy = ppy + (((float)j) + 0.5f) * K.pdy;
+  const float *C = P.cam;
   if (P.lens == DR_SPHERICAL_SEGMENT_LENS) {
     // Lenses::SphericalSegmentLense (SphericalSegmentLense.impl.cuh:13-30): jittered angles
     // alpha * position / RADIAN, target = direction * distance, origin on a disc of radius
     // asin(u) * sensor; the origin goes through the camera's point map, the unit direction
     // (target - origin) through Transform::direction.
-    const float offx = pdx * 0.5f, offy = pdy * 0.5f;
-    const float ha = ((sx + ((u0 * (offx - (-offx))) + (-offx))) * P.lens_p[0]) / 57.29577951308233f;
-    const float va = ((sy + ((u1 * (offy - (-offy))) + (-offy))) * P.lens_p[1]) / 57.29577951308233f;
+    const float ha = ((sx + ((u0 * K.jx) + K.ox)) * P.lens_p[0]) / 57.29577951308233f;
+    const float va = ((sy + ((u1 * K.jy) + K.oy)) * P.lens_p[1]) / 57.29577951308233f;
     float hs, hc, vs, vc;
     sincos_rad(ha, hs, hc);
     sincos_rad(va, vs, vc);
@@ -136,27 +175,27 @@ DR_HD void camera_ray(const dr_render_params &P, int px, int py, int i, int j, f
     sincos_2pi(u2, rs, rc);                     // range(0, 2*PI)
     const V3 org = (v3(rc, rs, 0.0f) * asin_unit(u3)) * P.lens_p[3];
     const V3 dl = normalized(target - org);
-    const float *R = P.cam_rows, *C = P.cam;
+    const float *R = P.cam_rows;
     o = v3(((org.x * R[0] + org.y * R[1]) + org.z * R[2]) + C[0], ((org.x * R[3] + org.y * R[4]) + org.z * R[5]) + C[1],
            ((org.x * R[6] + org.y * R[7]) + org.z * R[8]) + C[2]);
     d = v3((dl.x * C[3] + dl.y * C[6]) + dl.z * C[9], (dl.x * C[4] + dl.y * C[7]) + dl.z * C[10],
            (dl.x * C[5] + dl.y * C[8]) + dl.z * C[11]);
     return;
   }
+  o = v3(C[0], C[1], C[2]);
   if (P.lens == DR_PERSPECTIVE_LENS) {
     // DefaultPerspectiveLense (DefaultPerspectiveLense.impl.cuh:12-15): subpixel centre,
-    // direction normalized in lens space before the camera transform
-    const V3 n = normalized(v3(sx, sy, P.cam[12]));
-    lx = n.x; ly = n.y; lz = n.z;
+    // direction normalized in lens space before the camera transform (Vector3::normalized
+    // of (sx, sy, cam[12]))
+    const float il = 1.0f / __builtin_sqrtf(sx * sx + sy * sy + K.zc[3]);
+    const float lx = sx * il, ly = sy * il, lz = C[12] * il;
+    d = v3((lx * C[3] + ly * C[6]) + lz * C[9], (lx * C[4] + ly * C[7]) + lz * C[10],
+           (lx * C[5] + ly * C[8]) + lz * C[11]);
   } else {
-    const float offx = pdx * 0.5f, offy = pdy * 0.5f;
-    lx = sx + ((u0 * (offx - (-offx))) + (-offx));
-    ly = sy + ((u1 * (offy - (-offy))) + (-offy));
-    lz = P.cam[12];
+    const float lx = sx + ((u0 * K.jx) + K.ox);
+    const float ly = sy + ((u1 * K.jy) + K.oy);
+    d = v3((lx * C[3] + ly * C[6]) + K.zc[0], (lx * C[4] + ly * C[7]) + K.zc[1], (lx * C[5] + ly * C[8]) + K.zc[2]);
   }
-  const float *C = P.cam;
-  o = v3(C[0], C[1], C[2]);
-  d = v3((lx * C[3] + ly * C[6]) + lz * C[9], (lx * C[4] + ly * C[7]) + lz * C[10], (lx * C[5] + ly * C[8]) + lz * C[11]);
 }
 
 // Unit geometric normal n = (b-a)x(c-a) normalised, turned to face the incoming ray.
@@ -231,10 +270,10 @@ DR_HD bool light_photon(const dr_render_params &P, const dr_light &L, int l, uin
   return true;
 }
 
-// BlockBasedFrameBuffer<8,16>::blockPixelLocation (BlockBasedFrameBuffer.impl.cuh:125-143)
-DR_HD bool block_pixel(int width, int height, int block, int pixel, int &x, int &y) {
-  const int nbw = (width + 8 - 1) / 8;
-  const int by = block / nbw, bx = block - by * nbw;
+// BlockBasedFrameBuffer<8,16>::blockPixelLocation (BlockBasedFrameBuffer.impl.cuh:125-143);
+// K = frame_k of the request.
+DR_HD bool block_pixel(const FrameK &K, int width, int height, int block, int pixel, int &x, int &y) {
+  const int by = (int)udiv((uint32_t)block, K.nbw), bx = block - by * (int)K.nbw.d;
   const int oy = pixel / 8, ox = pixel - oy * 8;
   x = 8 * bx + ox;
   y = 16 * by + oy;
