@@ -24,11 +24,13 @@ constexpr int kRB = 256;               // lanes per workgroup
 constexpr int kRoundSamples = 16;      // samples per pixel staged in LDS per round
 constexpr int kBlockPixels = 128;      // BlockBasedFrameBuffer<8,16>
 constexpr int kLeafMin = 16;           // pending leaves that trigger the (wave-shared) leaf tests
-constexpr int kShadeMin = 32;          // finished casts that trigger the path updates
+constexpr int kShadeMin = 32;          // finished casts that trigger the adjoint's path updates
 // Idle lanes a wave collects before it starts new path jobs (1, 8, 16, 32 measured
 // 34.9 / 35.0 / 35.3 / 38.4 ms at 1080p 16 spp, 0.84 / 0.77 / 0.80 / 0.90 ms at 256x256
 // 4 spp, 15.7 / 15.3 / 15.6 / 16.3 ms for the example scene at 1 spp, one block per
-// workgroup; 4, 8, 16 gave 29.8 / 29.8 / 30.4 ms at 1080p on the streamed kernel).
+// workgroup; 4, 8, 16 gave 29.8 / 29.8 / 30.4 ms at 1080p on the streamed kernel).  Used by the
+// adjoint and as the batch autotuner's starting point; the render kernels take the estimator's
+// kRefillMin / kStreamRefillMin.
 constexpr int kRenderRefillMin = 8;
 // Waves per SIMD the render kernels are compiled for: the register cap this sets
 // (168 VGPRs) costs a few spills but beats 2 waves per SIMD clearly.
@@ -155,6 +157,11 @@ struct BenchPath {
   // back from the block ring when its pixel is written (29.5 ms at 1080p 16 spp, against
   // 30.8 ms with the other choices, which suit RefPath)
   static constexpr bool kStreamBlackInUpdate = false, kStreamBlockInRegister = false;
+  // Finished casts that trigger the path updates and idle lanes that trigger new path jobs, in
+  // render_block and in k_render_stream (1080p 16 spp, split 8 / depth 24: 27.2 ms at 32 / 8,
+  // 26.4-26.5 ms at 24 or 20 / 4 in the streamed kernel; 0.67 ms at 32 / 8 against 0.70 ms at
+  // 24 / 4 for the 256x256 frame in render_block)
+  static constexpr int kShadeMin = 32, kRefillMin = 8, kStreamShadeMin = 24, kStreamRefillMin = 4;
   Path st;
   __device__ __forceinline__ bool start(const dr_render_params &P, const FrameK &K, int block, int lp, int s,
                                         uint32_t samp0) {
@@ -217,6 +224,9 @@ struct RefPath {
   // job's block stays in a register (example scene 8.5 ms, against 9.4-9.9 ms otherwise:
   // fewer spilled registers)
   static constexpr bool kStreamBlackInUpdate = true, kStreamBlockInRegister = true;
+  // the same thresholds (example scene, streamed: 8.12 ms at 32 / 8, 8.01 at 44 / 8, 7.94 at
+  // 44 / 4, 9.37 at 64 / 8)
+  static constexpr int kShadeMin = 32, kRefillMin = 8, kStreamShadeMin = 44, kStreamRefillMin = 4;
   uint32_t pix, samp;
   int layer, phase, light, lk, excl, trans;   // lk: photons of the current light still to send
   V3 o, d;                          // the next ray to cast
@@ -523,7 +533,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
         walk_leaf_wave(T, w, pend, clip, lds.best, kStats ? lds.ins : nullptr, cc);
       // ---- path updates of the finished casts, batched
       const uint64_t dm = __ballot(job >= 0 && w.state == kWalkDone);
-      if (dm && (__popcll(dm) >= kShadeMin || __ballot(job >= 0 && w.state != kWalkDone) == 0)) {
+      if (dm && (__popcll(dm) >= PathT::kShadeMin || __ballot(job >= 0 && w.state != kWalkDone) == 0)) {
         if (job >= 0 && w.state == kWalkDone) {
           if (ps.shade(P, tb, w)) {
             finish(ps.color());
@@ -559,7 +569,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
                       best + (threadIdx.x & ~63u), kStats ? ins + (threadIdx.x & ~63u) : nullptr};
   RenderCounters rc;
   const FrameK K = frame_k_uniform(P);
-  render_block<PathT, kStats>(T, tb, P, K, frame, start_block + (int)blockIdx.x, L, kRenderRefillMin, &rc);
+  render_block<PathT, kStats>(T, tb, P, K, frame, start_block + (int)blockIdx.x, L, PathT::kRefillMin, &rc);
   if (kStats) flush_counters(rc, counts);
 }
 
@@ -672,7 +682,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
     // ---- idle lanes take the next jobs of the stream
     const bool want = (q < 0) && !exhausted;
     const uint64_t wmask = __ballot(want);
-    if (wmask && (__popcll(wmask) >= kRenderRefillMin || __ballot(live) == 0)) {
+    if (wmask && (__popcll(wmask) >= PathT::kStreamRefillMin || __ballot(live) == 0)) {
       const int leader = __ffsll((unsigned long long)wmask) - 1;
       unsigned base = 0;
       if (lane == leader) base = atomicAdd(&next_job, (unsigned)__popcll(wmask));
@@ -739,7 +749,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
       walk_leaf_wave(T, w, pend, clip, best, ins, cc);
     // ---- path updates of the finished casts, batched
     const uint64_t dm = __ballot(live && w.state == kWalkDone);
-    if (dm && (__popcll(dm) >= kShadeMin || __ballot(live && w.state != kWalkDone) == 0)) {
+    if (dm && (__popcll(dm) >= PathT::kStreamShadeMin || __ballot(live && w.state != kWalkDone) == 0)) {
       if (live && w.state == kWalkDone) {
         if ((PathT::kStreamBlackInUpdate && nopath) || ps.shade(P, tb, w)) {
           const float *col = ps.color();
