@@ -139,9 +139,15 @@ __device__ __forceinline__ void group_entry(const float4 *__restrict__ G, int g,
   // children with no non-empty leaf below them are dropped before their box tests run
   const uint2 head = *(const uint2 *)(q + kGroupMask / 4);
   uint32_t m = 0;
+  // all 192 B of child boxes requested at once, before any child is tested: the loads of the
+  // second four children no longer wait behind the tests of the first four (incoherent cast
+  // 2.41 -> 2.34 ms, 1080p render 26.3 -> 26.0 ms)
+  float4 qv[12];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {       // two halves of 4 children keep register use low
-    const float4 sx = q[0 + h], sy = q[2 + h], sz = q[4 + h], ex = q[6 + h], ey = q[8 + h], ez = q[10 + h];
+  for (int k = 0; k < 12; ++k) qv[k] = q[k];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {       // children in two halves of 4
+    const float4 sx = qv[0 + h], sy = qv[2 + h], sz = qv[4 + h], ex = qv[6 + h], ey = qv[8 + h], ez = qv[10 + h];
     const float a_sx[4] = {sx.x, sx.y, sx.z, sx.w}, a_sy[4] = {sy.x, sy.y, sy.z, sy.w},
                 a_sz[4] = {sz.x, sz.y, sz.z, sz.w}, a_ex[4] = {ex.x, ex.y, ex.z, ex.w},
                 a_ey[4] = {ey.x, ey.y, ey.z, ey.w}, a_ez[4] = {ez.x, ez.y, ez.z, ez.w};
