@@ -59,7 +59,8 @@ __device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ UDiv uni(UDiv D) { return {uni(D.d), uni(D.m)}; }
 __device__ __forceinline__ UDiv udiv_uniform(uint32_t d) { return uni(make_udiv(d)); }
-// frame_k(P) in scalar registers, formed before a path loop over P's blocks.
+// frame_k(P) formed in the kernel and moved to scalar registers (the kernels otherwise take
+// the host's frame_k as an argument).
 __device__ __forceinline__ FrameK frame_k_uniform(const dr_render_params &P) {
   FrameK K = frame_k(P);
   K.nbw = uni(K.nbw);
@@ -76,6 +77,7 @@ __device__ __forceinline__ FrameK frame_k_uniform(const dr_render_params &P) {
   for (int k = 0; k < 4; k++) K.zc[k] = uni(K.zc[k]);
   K.amount = uni(K.amount);
   K.keep = uni(K.keep);
+  K.inv_rr = uni(K.inv_rr);
   return K;
 }
 
@@ -120,8 +122,8 @@ __device__ __forceinline__ bool next_photon(const dr_render_params &P, Path &st,
 }
 
 // Vertex setup after a geometry hit of the ray with direction `dir`.
-__device__ __forceinline__ void setup_vertex(const dr_render_params &P, const float4 *__restrict__ shade, const Hit &hh,
-                                             V3 dir, Path &st) {
+__device__ __forceinline__ void setup_vertex(const dr_render_params &P, const FrameK &K, const float4 *__restrict__ shade,
+                                             const Hit &hh, V3 dir, Path &st) {
   st.h = hh.id;
   st.p = hh.p;
   const float4 sh = shade[hh.id];
@@ -129,7 +131,7 @@ __device__ __forceinline__ void setup_vertex(const dr_render_params &P, const fl
   st.kind = P.materials[st.mid].kind;
   const V3 du = normalized(dir);
   st.nu = facing_normal(v3(sh.x, sh.y, sh.z), du);
-  st.has_sample = next_sample(P, st.kind, st.pix, st.samp, st.layer, st.nu, du, st.sdir, st.sweight);
+  st.has_sample = next_sample(P, K, st.kind, st.pix, st.samp, st.layer, st.nu, du, st.sdir, st.sweight);
   st.light = -1;
 }
 
@@ -162,6 +164,7 @@ struct BenchPath {
   // 26.4-26.5 ms at 24 or 20 / 4 in the streamed kernel; 0.67 ms at 32 / 8 against 0.70 ms at
   // 24 / 4 for the 256x256 frame in render_block)
   static constexpr int kShadeMin = 32, kRefillMin = 8, kStreamShadeMin = 24, kStreamRefillMin = 4;
+  static constexpr bool kStreamFrameInKernel = false;
   Path st;
   __device__ __forceinline__ bool start(const dr_render_params &P, const FrameK &K, int block, int lp, int s,
                                         uint32_t samp0) {
@@ -188,11 +191,11 @@ struct BenchPath {
   // After the lane's cast ended.  A geometry hit sets up the vertex, then photons
   // (Lambert) or the next bounce; a photon adds its contribution if it reached the
   // vertex, then the next photon.  Returns true when the path is done.
-  __device__ __forceinline__ bool shade(const dr_render_params &P, const Tables &tb, const Walk &w) {
+  __device__ __forceinline__ bool shade(const dr_render_params &P, const FrameK &K, const Tables &tb, const Walk &w) {
     bool photons = false;
     if (st.phase == 0) {
       if (!w.hit) return true;
-      setup_vertex(P, tb.shade, w.h, w.ray.d, st);
+      setup_vertex(P, K, tb.shade, w.h, w.ray.d, st);
       photons = st.kind == DR_LAMBERT;
     } else {
       const V3 e = w.h.p - st.p;
@@ -227,6 +230,7 @@ struct RefPath {
   // the same thresholds (example scene, streamed: 8.12 ms at 32 / 8, 8.01 at 44 / 8, 7.94 at
   // 44 / 4, 9.37 at 64 / 8)
   static constexpr int kShadeMin = 32, kRefillMin = 8, kStreamShadeMin = 44, kStreamRefillMin = 4;
+  static constexpr bool kStreamFrameInKernel = true;
   uint32_t pix, samp;
   int layer, phase, light, lk, excl, trans;   // lk: photons of the current light still to send
   V3 o, d;                          // the next ray to cast
@@ -293,7 +297,7 @@ struct RefPath {
   }
   // The indirect sample of the vertex and the child layer's ray and throughput; false when
   // the path ends here.
-  __device__ __forceinline__ bool next_layer(const dr_render_params &P, const Tables &tb) {
+  __device__ __forceinline__ bool next_layer(const dr_render_params &P, const FrameK &K, const Tables &tb) {
     if (layer >= P.max_bounces) return false;
     const dr_material M = P.materials[mid];
     float u[4];
@@ -301,7 +305,7 @@ struct RefPath {
     float sw = 1.0f;
     if (layer >= P.rr_start) {
       if (u[2] >= P.rr_survive) return false;
-      sw = 1.0f / P.rr_survive;
+      sw = K.inv_rr;
     }
     const RefHit H = hit_record(P, tb, M);
     V3 sdir;
@@ -333,7 +337,7 @@ struct RefPath {
     return true;
   }
   // After the lane's cast ended; true when the path is done.
-  __device__ __forceinline__ bool shade(const dr_render_params &P, const Tables &tb, const Walk &w) {
+  __device__ __forceinline__ bool shade(const dr_render_params &P, const FrameK &K, const Tables &tb, const Walk &w) {
     if (phase == 0) {
       if (!w.hit) return true;                        // the layer ray missed (:418-421)
       h = w.h.id;
@@ -393,7 +397,7 @@ struct RefPath {
       phase = 1;
       return false;
     }
-    return !next_layer(P, tb);
+    return !next_layer(P, K, tb);
   }
 };
 
@@ -457,7 +461,7 @@ __device__ __forceinline__ void store_pixel(const dr_render_params &P, const Fra
 }
 
 // Renders framebuffer block `block` of one request (all lanes of the workgroup call it),
-// in rounds of kRoundSamples samples per pixel.  K = frame_k_uniform(P).
+// in rounds of kRoundSamples samples per pixel.  K = frame_k(P).
 // refill_min: an idle lane takes a new job once at least this many lanes of its wave
 // are idle (or the whole wave is).
 template <class PathT, bool kStats = false>
@@ -535,7 +539,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
       const uint64_t dm = __ballot(job >= 0 && w.state == kWalkDone);
       if (dm && (__popcll(dm) >= PathT::kShadeMin || __ballot(job >= 0 && w.state != kWalkDone) == 0)) {
         if (job >= 0 && w.state == kWalkDone) {
-          if (ps.shade(P, tb, w)) {
+          if (ps.shade(P, K, tb, w)) {
             finish(ps.color());
           } else {
             walk_begin(T, w, ps.ray_o(), ps.ray_d(), ps.ray_excl(), lds.stk, kRB, nullptr);
@@ -557,7 +561,7 @@ __device__ __forceinline__ void render_block(const DevTree &T, const Tables &tb,
 
 // One framebuffer block per workgroup (sample counts above the streamed kernel's limit).
 template <class PathT, bool kStats = false>
-__global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables tb, dr_render_params P,
+__global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables tb, dr_render_params P, FrameK K,
                                                                float *__restrict__ frame, int start_block, int levels,
                                                                unsigned long long *__restrict__ counts = nullptr) {
   extern __shared__ uint32_t lds[];
@@ -568,7 +572,6 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render(DevTree T, Tables 
   const BlockLds L = {lds + threadIdx.x, (float *)(lds + levels * kStackWordsPerLevel * kRB), acc, &next_job,
                       best + (threadIdx.x & ~63u), kStats ? ins + (threadIdx.x & ~63u) : nullptr};
   RenderCounters rc;
-  const FrameK K = frame_k_uniform(P);
   render_block<PathT, kStats>(T, tb, P, K, frame, start_block + (int)blockIdx.x, L, PathT::kRefillMin, &rc);
   if (kStats) flush_counters(rc, counts);
 }
@@ -609,7 +612,7 @@ __host__ __device__ constexpr int stream_slots(int spp) {
 
 template <class PathT, bool kStats>
 __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, Tables tb, dr_render_params P,
-                                                                      float *__restrict__ frame, int start_block,
+                                                                      FrameK K, float *__restrict__ frame, int start_block,
                                                                       int end_block, int levels,
                                                                       unsigned *__restrict__ work,
                                                                       unsigned long long *__restrict__ counts) {
@@ -638,7 +641,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
   const int nrange = end_block - start_block;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
   const float spw = uni(1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
-  const FrameK K = frame_k_uniform(P);
+  if constexpr (PathT::kStreamFrameInKernel) K = frame_k_uniform(P);
   const UDiv Dspp = udiv_uniform((uint32_t)spp), Dslots = udiv_uniform((uint32_t)nslots);
   const bool clip = P.cull_backfaces != 0;
   const int lane = (int)__lane_id();
@@ -751,7 +754,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_render_stream(DevTree T, 
     const uint64_t dm = __ballot(live && w.state == kWalkDone);
     if (dm && (__popcll(dm) >= PathT::kStreamShadeMin || __ballot(live && w.state != kWalkDone) == 0)) {
       if (live && w.state == kWalkDone) {
-        if ((PathT::kStreamBlackInUpdate && nopath) || ps.shade(P, tb, w)) {
+        if ((PathT::kStreamBlackInUpdate && nopath) || ps.shade(P, K, tb, w)) {
           const float *col = ps.color();
           float c0 = col[0], c1 = col[1], c2 = col[2];
           if (P.lens == DR_SPHERICAL_SEGMENT_LENS) {
@@ -861,7 +864,8 @@ __device__ __forceinline__ void wave_add3(float *acc, bool has, int key, const f
 }
 
 __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const float4 *__restrict__ shade,
-                                                                 dr_render_params P, const float *__restrict__ dl,
+                                                                 dr_render_params P, FrameK FK,
+                                                                 const float *__restrict__ dl,
                                                                  double *__restrict__ g_albedo,
                                                                  double *__restrict__ g_emission,
                                                                  float *__restrict__ vtx_all,
@@ -878,7 +882,6 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
   const int spp = P.fsaa_x * P.fsaa_y;
   const uint32_t samp0 = (uint32_t)(P.iteration > 1 ? P.iteration - 1 : 0) * (uint32_t)spp;
   const float spw = uni(1.0f / (((float)P.fsaa_x) * ((float)P.fsaa_y)));
-  const FrameK FK = frame_k_uniform(P);
   const UDiv Dspp = udiv_uniform((uint32_t)spp);
   const bool clip = P.cull_backfaces != 0;
   const int lane = (int)__lane_id();
@@ -968,7 +971,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_backward(DevTree T, const
           if (st.phase == 0) {
             if (!w.hit) fin = true;
             else {
-              setup_vertex(P, shade, w.h, w.ray.d, st);
+              setup_vertex(P, FK, shade, w.h, w.ray.d, st);
               V_(K, 0) = __int_as_float(st.mid);
               for (int c = 0; c < 4; c++) V_(K, 4 + c) = 0.0f;
               K++;
@@ -1047,6 +1050,7 @@ struct DevRequest {
   const float4 *shade;
   const float *faces;            // shading records of the reference plugins
   dr_render_params P;
+  FrameK K;                      // frame_k(P), formed on the host
   float *frame;
 };
 
@@ -1075,8 +1079,7 @@ __global__ __launch_bounds__(kRB, kRenderWaves) void k_batch(const DevRequest *_
     const int4 w = items[it];
     const DevRequest &R = reqs[w.x];
     const Tables tb = {R.shade, R.faces};
-    const FrameK K = frame_k_uniform(R.P);
-    for (int b = w.y; b < w.y + w.z; b++) render_block<PathT>(R.T, tb, R.P, K, R.frame, b, L, refill_min);
+    for (int b = w.y; b < w.y + w.z; b++) render_block<PathT>(R.T, tb, R.P, R.K, R.frame, b, L, refill_min);
   }
 }
 
@@ -1141,8 +1144,8 @@ static int launch_render(dr_scene s, const dr_render_params *p, float *frame, in
     const int groups = (int)std::min<long long>(nrange, resident);
     if (nrange >= 2 * resident) {
       unsigned *work = (unsigned *)s->work.p + 2 * (s->work_slot.fetch_add(1) % kWorkSlots);
-      hipLaunchKernelGGL(kern, dim3(groups), dim3(kRB), lds_s, (hipStream_t)stream, s->tree, tb, *p, frame,
-                         start_block, end_block, levels, work, counts);
+      hipLaunchKernelGGL(kern, dim3(groups), dim3(kRB), lds_s, (hipStream_t)stream, s->tree, tb, *p, frame_k(*p),
+                         frame, start_block, end_block, levels, work, counts);
       DR_HIP_CHECK(hipGetLastError());
       return DR_OK;
     }
@@ -1153,7 +1156,7 @@ static int launch_render(dr_scene s, const dr_render_params *p, float *frame, in
   if (lds > 64 * 1024)
     DR_HIP_CHECK(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kernel, dim3(end_block - start_block), dim3(kRB), lds, (hipStream_t)stream, s->tree, tb, *p,
-                     frame, start_block, levels, counts);
+                     frame_k(*p), frame, start_block, levels, counts);
   DR_HIP_CHECK(hipGetLastError());
   return DR_OK;
 }
@@ -1213,8 +1216,8 @@ static void launch_backward(dr_scene s, const dr_render_params *p, const float *
   int *counter = (int *)scratch_mem;
   float *vtx = (float *)((char *)scratch_mem + kCounterBytes);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, counter, 1);
-  hipLaunchKernelGGL(k_backward, dim3(groups), dim3(kRB), lds, stream, s->tree, tables_of(s, p).shade, *p, dl, ga, ge,
-                     vtx, counter, nblocks, levels, maxv);
+  hipLaunchKernelGGL(k_backward, dim3(groups), dim3(kRB), lds, stream, s->tree, tables_of(s, p).shade, *p, frame_k(*p),
+                     dl, ga, ge, vtx, counter, nblocks, levels, maxv);
 }
 
 }  // namespace dr
@@ -1384,6 +1387,7 @@ static int autotune(const std::vector<DevRequest> &R, int levels, int num_cu, hi
   for (size_t r = 0; r < R.size() && rc == DR_OK; r++) {
     proxy[r].P.fsaa_x = proxy[r].P.fsaa_y = 1;
     proxy[r].P.iteration = 1;
+    proxy[r].K = frame_k(proxy[r].P);
     frames[r].bytes = (size_t)block_count(&R[r].P) * kBlockPixels * 16;
     hipError_t e = hipMalloc(&frames[r].p, frames[r].bytes);
     if (e != hipSuccess) { frames[r].p = nullptr; rc = fail(DR_ERR_ALLOC, "dr_batch_create: scratch frames"); }
@@ -1441,6 +1445,7 @@ extern "C" int dr_batch_create(const dr_request *reqs, int n, const dr_batch_tun
     d.shade = tb.shade;
     d.faces = tb.faces;
     d.P = reqs[r].params;
+    d.K = dr::frame_k(d.P);
     dr::scene_retain(reqs[r].scene);
     b->scenes.push_back(reqs[r].scene);
     if (reqs[r].params.shading) {

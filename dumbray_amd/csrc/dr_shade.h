@@ -116,9 +116,10 @@ DR_HD uint32_t udiv(uint32_t n, UDiv D) {
   return (n - q * D.d >= D.d) ? q + 1 : q;
 }
 
-// Values of one request's camera and framebuffer mapping that depend on the parameters
-// alone.  The kernels form them once, before the path loop, and keep them in scalar
-// registers; formed inside the loop, they would be hoisted out of it into vector registers.
+// Values of one request's camera, framebuffer mapping and Russian roulette that depend on
+// the parameters alone.  The host forms them (frame_k, the same fp32 expressions the kernels
+// would evaluate) and passes them to the kernels, where they sit in scalar registers; formed
+// inside the path loop, the compiler hoists them out of it into vector registers.
 struct FrameK {
   UDiv nbw;              // framebuffer blocks per row
   UDiv fy;               // fsaa_y: sample s is subpixel (s / fsaa_y, s % fsaa_y)
@@ -127,6 +128,7 @@ struct FrameK {
   float jx, ox, jy, oy;  // lens jitter: range offx - (-offx) and start -offx (x), the same for y
   float zc[4];           // cam[12] * cam[9], cam[10], cam[11], cam[12]
   float amount, keep;    // blend weight of this iteration (1 / iteration) and 1 - amount
+  float inv_rr;          // 1 / rr_survive: the weight of a path that survives Russian roulette
 };
 DR_HD FrameK frame_k(const dr_render_params &P) {
   FrameK K;
@@ -146,6 +148,7 @@ DR_HD FrameK frame_k(const dr_render_params &P) {
   for (int k = 0; k < 4; k++) K.zc[k] = P.cam[12] * P.cam[9 + k];
   K.amount = (P.iteration <= 1) ? 1.0f : (1.0f / ((float)P.iteration));
   K.keep = 1.0f - K.amount;
+  K.inv_rr = 1.0f / P.rr_survive;
   return K;
 }
 
@@ -225,16 +228,16 @@ DR_HD V3 reflect_unit(V3 v, V3 n) {
   return v3(v.x - (n.x * mul), v.y - (n.y * mul), v.z - (n.z * mul));
 }
 
-// Sampling decision at `layer` (requestIndirectSamples + Russian roulette).
-DR_HD bool next_sample(const dr_render_params &P, int kind, uint32_t pix, uint32_t samp, int layer, V3 nu, V3 du,
-                       V3 &dir, float &weight) {
+// Sampling decision at `layer` (requestIndirectSamples + Russian roulette); K = frame_k(P).
+DR_HD bool next_sample(const dr_render_params &P, const FrameK &K, int kind, uint32_t pix, uint32_t samp, int layer,
+                       V3 nu, V3 du, V3 &dir, float &weight) {
   if (layer >= P.max_bounces) return false;
   float u[4];
   rng4(P, pix, samp, (uint32_t)layer, 1, u);
   weight = 1.0f;
   if (layer >= P.rr_start) {
     if (u[2] >= P.rr_survive) return false;
-    weight = 1.0f / P.rr_survive;
+    weight = K.inv_rr;
   }
   dir = (kind == DR_LAMBERT) ? cosine_dir(nu, u[0], u[1]) : reflect_unit(du, nu);
   return true;
