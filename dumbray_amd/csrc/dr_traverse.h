@@ -364,12 +364,15 @@ __device__ __forceinline__ void walk_leaf_wave(const DevTree &T, Walk &w, bool p
     const float t = __uint_as_float((unsigned)(key >> 32));
     const int k = w.llink + (int)(key & 0xffffffffu);
     const V3 p = w.ray.o + w.ray.d * t;
+    // the record's id is read beside the leaf box, before the box test that decides whether
+    // it is needed
+    const int rid = kIndexed ? __float_as_int(T.urecs[4 * T.lref[k] + 3].w) : __float_as_int(T.recs[4 * k + 3].w);
     Box lb;
     if (w.lg < 0) { lb.s = v3(T.rsx, T.rsy, T.rsz); lb.e = v3(T.rex, T.rey, T.rez); }
     else lb = child_box(T.groups, w.lg, w.lk);
     if (contains(lb, p)) {
       hit = true;
-      w.h.id = kIndexed ? __float_as_int(T.urecs[4 * T.lref[k] + 3].w) : __float_as_int(T.recs[4 * k + 3].w);
+      w.h.id = rid;
       w.h.t = t;
       w.h.p = p;
     }
