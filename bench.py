@@ -559,8 +559,9 @@ def main():
                 "unit": "T lane-ops/s", "frac": round(ops / t / 1e12 / VALU_PEAK_TOPS, 4),
                 "traffic": None if traffic is None else int(traffic),
                 "kernel": "dr::k_cast<false>", "kernel_ms": round(kB_ms, 4), "ops_per_launch": int(ops),
-                "binding": "instruction issue: the useful lane-ops above are the kernel's box, plane and inside "
-                           "tests; the rest of the issue slots go to idle lanes and control",
+                "binding": "instruction issue, then load latency: the useful lane-ops above are the kernel's box, "
+                           "plane and inside tests; the rest of the issue slots go to idle lanes and control, and "
+                           "the time the VALU is idle (1 - valu_busy) to waits on dependent loads",
                 "counters": None if pmc is None else {k: (round(v, 4) if isinstance(v, float) else v)
                                                       for k, v in pmc.items()},
                 "counters_source": src,
